@@ -1,0 +1,1705 @@
+// des_amd.hip -- HIP kernels for gfx950 and the C-ABI (include/des_amd.h) of
+// the DynEarthSol explicit time step.
+//
+// Two ways through a step:
+//   * per-function kernels, one per reference function, in the order of the
+//     reference's main loop (des_step, dynearthsol.cxx:325-361);
+//   * the fused six-pass schedule (des_run): three element passes and three
+//     node passes per step, shape functions and strain rates recomputed in
+//     registers instead of stored, mass / volume_n / tmass built inside the
+//     node pass that first needs them, and the Jaumann rotation of step k
+//     applied when step k+1 first reads the stress.
+// Element->node sums are gathers over the node->element CSR in ascending
+// element order -- the reference's serial summation order -- so no atomics and
+// bitwise run-to-run determinism.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "des_amd.h"
+#include "des_device.hpp"
+
+using namespace des;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int RED_BLOCKS = 1024;
+
+inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+// ---------------------------------------------------------------- kernels
+
+// compute_volume (geometry.cxx:86-108)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_volume(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    s.volume[e] = elem_volume<ND>(x);
+}
+
+// compute_shape_fn (geometry.cxx:368-455)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_shape_fn(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND], sh[ND][NPE];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    shape_fn<ND>(x, s.volume[e], sh);
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) s.shp[(d * NPE + i) * (long)s.E + e] = sh[d][i];
+}
+
+template <int ND> __device__ __forceinline__ void load_shp(const DevState &s, int e,
+                                                         double sh[ND][Dim<ND>::NPE])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) sh[d][i] = s.shp[(d * NPE + i) * (long)s.E + e];
+}
+
+// element part of compute_mass (geometry.cxx:346-352): m and tm of element e
+template <int ND> __device__ __forceinline__ void mass_of(const DevState &s, int e, double vol,
+                                                        double Tmean, double &m, double &tm)
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    const des_params *P = s.P;
+    const double pseudo_speed = P->max_vbc_val * P->inertial_scaling;
+    double rho_e = mat_rho(s, e, Tmean);
+    double rho = P->is_quasi_static ? harmonic_mean(s, P->bulk_modulus, e) / (pseudo_speed * pseudo_speed)
+                                    : rho_e;
+    m = rho * vol / NPE;
+    tm = rho_e * arithmetic_mean(s, P->heat_capacity, e) * vol / NPE;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_mass_elem(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    int c[Dim<ND>::NPE];
+    load_conn<ND>(s, e, c);
+    double m, tm;
+    mass_of<ND>(s, e, s.volume[e], elem_mean<ND>(s.T, c), m, tm);
+    s.escal[1 * (long)s.E + e] = m;
+    s.escal[2 * (long)s.E + e] = tm;
+}
+
+// node part of compute_mass (geometry.cxx:353-359)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_mass_node(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    const int thermal = s.P->has_thermal_diffusion;
+    double vn = 0, m = 0, tm = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
+        int e = s.csr_ent[p] / NPE;
+        vn += s.volume[e];
+        m += s.escal[1 * (long)s.E + e];
+        if (thermal) tm += s.escal[2 * (long)s.E + e];
+    }
+    s.volume_n[n] = vn;
+    s.mass[n] = m;
+    s.tmass[n] = tm;
+}
+
+// thermal contribution of element e to tdot at its nodes (fields.cxx:110-142)
+template <int ND> __device__ __forceinline__ void temperature_contrib(const DevState &s, int e, double vol,
+                                                                    const double sh[ND][Dim<ND>::NPE],
+                                                                    const double Tn[Dim<ND>::NPE],
+                                                                    double out[Dim<ND>::NPE])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    const des_params *P = s.P;
+    double kv = arithmetic_mean(s, P->therm_cond, e) * vol;
+    double rh = (arithmetic_mean(s, P->radiogenic_heating, e) / arithmetic_mean(s, P->heat_capacity, e)) * vol;
+    double D[NPE][NPE];
+#pragma unroll
+    for (int i = 0; i < NPE; ++i)
+#pragma unroll
+        for (int j = 0; j < NPE; ++j) {
+            if constexpr (ND == 3) D[i][j] = (sh[0][i] * sh[0][j] + sh[1][i] * sh[1][j] + sh[2][i] * sh[2][j]);
+            else D[i][j] = (sh[0][i] * sh[0][j] + sh[1][i] * sh[1][j]);
+        }
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        double diffusion = 0;
+#pragma unroll
+        for (int j = 0; j < NPE; ++j) diffusion += D[i][j] * Tn[j];
+        out[i] = diffusion * kv - rh;
+    }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_elem(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double sh[ND][NPE], Tn[NPE], out[NPE];
+    load_shp<ND>(s, e, sh);
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) Tn[i] = s.T[c[i]];
+    temperature_contrib<ND>(s, e, s.volume[e], sh, Tn, out);
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) s.ework[e * NPE + i] = out[i];
+}
+
+// temperature update with its boundary conditions (fields.cxx:153-162)
+__device__ __forceinline__ double temperature_new(const DevState &s, int n, double T, double tdot,
+                                                  double dt, double tmass)
+{
+    const des_params *P = s.P;
+    unsigned f = s.bcflag[n];
+    if (f & BOUNDZ1) return P->surface_temperature;
+    if ((f & BOUNDZ0) && !P->has_fixed_bottom_temperature) return P->mantle_temperature;
+    return T - tdot * dt / tmass;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_node(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    double tdot = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) tdot += s.ework[s.csr_ent[p]];
+    s.ntmp[n] = tdot;
+    s.T[n] = temperature_new(s, n, s.T[n], tdot, s.sc->dt, s.tmass[n]);
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_strain_rate(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double v[NPE][ND], sh[ND][NPE], sr[NSTR];
+    gather_vec<ND>(s.vel, s.N, c, v);
+    load_shp<ND>(s, e, sh);
+    strain_rate<ND>(v, sh, sr);
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) s.strain_rate[k * (long)s.E + e] = sr[k];
+}
+
+// compute_dvoldt (geometry.cxx:111-155), into ntmp
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dvoldt_node(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    double sum = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
+        int e = s.csr_ent[p] / NPE;
+        double sr[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) sr[k] = s.strain_rate[k * (long)s.E + e];
+        double dj = trace<ND>(sr);
+        sum += dj * s.volume[e];
+    }
+    s.ntmp[n] = sum / s.volume_n[n];
+}
+
+// compute_edvoldt (geometry.cxx:158-179)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_edvoldt(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    int c[Dim<ND>::NPE];
+    load_conn<ND>(s, e, c);
+    double dj = 0;
+#pragma unroll
+    for (int i = 0; i < Dim<ND>::NPE; ++i) dj += s.ntmp[c[i]];
+    s.edvoldt[e] = dj / Dim<ND>::NPE;
+}
+
+template <int ND> __device__ __forceinline__ void load_tensor(const double *a, long E, int e,
+                                                            double t[Dim<ND>::NSTR])
+{
+#pragma unroll
+    for (int k = 0; k < Dim<ND>::NSTR; ++k) t[k] = a[k * E + e];
+}
+
+template <int ND> __device__ __forceinline__ void store_tensor(double *a, long E, int e,
+                                                             const double t[Dim<ND>::NSTR])
+{
+#pragma unroll
+    for (int k = 0; k < Dim<ND>::NSTR; ++k) a[k * E + e] = t[k];
+}
+
+__device__ __forceinline__ void set_error(DevScalars *sc, int code)
+{
+    atomicCAS(&sc->err, 0, code);
+}
+
+// update_stress (rheology.cxx:510-646)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_stress(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double st[NSTR], es[NSTR], edot[NSTR];
+    load_tensor<ND>(s.stress, E, e, st);
+    load_tensor<ND>(s.strain, E, e, es);
+    load_tensor<ND>(s.strain_rate, E, e, edot);
+    double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
+    double Tm = elem_mean<ND>(s.T, c);
+    bool ok = stress_update<ND>(s, e, s.sc->dt, s.edvoldt[e], Tm, s.volume[e], s.volume_old[e],
+                                st, syy, es, edot, pl, dpl, dp);
+    if (!ok) set_error(s.sc, DES_ERR_INPUT);
+    store_tensor<ND>(s.stress, E, e, st);
+    store_tensor<ND>(s.strain, E, e, es);
+    store_tensor<ND>(s.strain_rate, E, e, edot);
+    s.stressyy[e] = syy;
+    s.plstrain[e] = pl;
+    s.delta_plstrain[e] = dpl;
+    s.dpressure[e] = dp;
+}
+
+// NMD_stress (geometry.cxx:182-236): node average of dp into ntmp ...
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_nmd_node(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    double sum = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
+        int e = s.csr_ent[p] / NPE;
+        sum += s.dpressure[e] * s.volume[e];
+    }
+    s.ntmp[n] = sum / s.volume_n[n];
+}
+
+// ... and the element correction of the stress diagonal
+template <int ND> __device__ __forceinline__ void nmd_correct(const DevState &s, int e, const int *c,
+                                                            double *st)
+{
+    double dp = 0;
+#pragma unroll
+    for (int i = 0; i < Dim<ND>::NPE; ++i) dp += s.ntmp[c[i]];
+    double dp_el = dp / Dim<ND>::NPE;
+    double dp_orig = s.dpressure[e];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) st[i] += (-dp_orig + dp_el) / ND;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_nmd_elem(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    int c[Dim<ND>::NPE];
+    load_conn<ND>(s, e, c);
+    double st[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) st[i] = s.stress[i * (long)s.E + e];
+    nmd_correct<ND>(s, e, c, st);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s.stress[i * (long)s.E + e] = st[i];
+}
+
+// element part of update_force (fields.cxx:304-330): what element e takes
+// away from the force of its nodes
+template <int ND> __device__ __forceinline__ void force_contrib(const DevState &s, int e, double vol,
+                                                              const double sh[ND][Dim<ND>::NPE],
+                                                              const double *st, double Tmean,
+                                                              double out[Dim<ND>::NPE][ND])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    const double gravity = s.P->gravity;
+    double buoy = 0;
+    if (gravity != 0) buoy = mat_rho(s, e, Tmean) * gravity / NPE;
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        if constexpr (ND == 3) {
+            out[i][0] = (st[0] * sh[0][i] + st[3] * sh[1][i] + st[4] * sh[2][i]) * vol;
+            out[i][1] = (st[3] * sh[0][i] + st[1] * sh[1][i] + st[5] * sh[2][i]) * vol;
+            out[i][2] = (st[4] * sh[0][i] + st[5] * sh[1][i] + st[2] * sh[2][i] + buoy) * vol;
+        } else {
+            out[i][0] = (st[0] * sh[0][i] + st[2] * sh[1][i]) * vol;
+            out[i][1] = (st[2] * sh[0][i] + st[1] * sh[1][i] + buoy) * vol;
+        }
+    }
+}
+
+template <int ND> __device__ __forceinline__ void store_force_contrib(const DevState &s, int e,
+                                                                    const double out[Dim<ND>::NPE][ND])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    double *w = s.ework + (long)e * NPE * ND;
+    if constexpr (ND == 3) {
+        double2 *w2 = reinterpret_cast<double2 *>(w);
+        w2[0] = make_double2(out[0][0], out[0][1]);
+        w2[1] = make_double2(out[0][2], out[1][0]);
+        w2[2] = make_double2(out[1][1], out[1][2]);
+        w2[3] = make_double2(out[2][0], out[2][1]);
+        w2[4] = make_double2(out[2][2], out[3][0]);
+        w2[5] = make_double2(out[3][1], out[3][2]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NPE; ++i)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) w[i * ND + d] = out[i][d];
+    }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_elem(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double sh[ND][NPE], st[NSTR], out[NPE][ND];
+    load_shp<ND>(s, e, sh);
+    load_tensor<ND>(s.stress, s.E, e, st);
+    force_contrib<ND>(s, e, s.volume[e], sh, st, elem_mean<ND>(s.T, c), out);
+    store_force_contrib<ND>(s, e, out);
+}
+
+// PREM pressure tables (matprops.cxx:11-99)
+__constant__ double c_prem_depth[46] = {
+    0e3, 3e3, 15e3, 24.4e3, 40e3, 60e3, 80e3, 115e3, 150e3, 185e3, 220e3, 265e3, 310e3, 355e3,
+    400e3, 450e3, 500e3, 550e3, 600e3, 635e3, 670e3, 721e3, 771e3, 871e3, 971e3, 1071e3,
+    1171e3, 1271e3, 1371e3, 1471e3, 1571e3, 1671e3, 1771e3, 1871e3, 1971e3, 2071e3, 2171e3,
+    2271e3, 2371e3, 2471e3, 2571e3, 2671e3, 2741e3, 2771e3, 2871e3, 2891e3};
+__constant__ double c_prem_p[2][46] = {
+    {0e8, 0.3e8, 3.3e8, 6.0e8, 11.2e8, 17.8e8, 24.5e8, 36.1e8, 47.8e8, 59.4e8, 71.1e8, 86.4e8,
+     102.0e8, 117.7e8, 133.5e8, 152.2e8, 171.3e8, 190.7e8, 210.4e8, 224.3e8, 238.3e8, 260.7e8,
+     282.9e8, 327.6e8, 372.8e8, 418.6e8, 464.8e8, 511.6e8, 558.9e8, 606.8e8, 655.2e8, 704.1e8,
+     753.5e8, 803.6e8, 854.3e8, 905.6e8, 957.6e8, 1010.3e8, 1063.8e8, 1118.2e8, 1173.4e8,
+     1229.7e8, 1269.7e8, 1287.0e8, 1345.6e8, 1357.5e8},
+    {0e8, 0.82e8, 4.1e8, 6.7e8, 11.2e8, 17.8e8, 24.5e8, 36.1e8, 47.8e8, 59.4e8, 71.1e8, 86.4e8,
+     102.0e8, 117.7e8, 133.5e8, 152.2e8, 171.3e8, 190.7e8, 210.4e8, 224.3e8, 238.3e8, 260.7e8,
+     282.9e8, 327.6e8, 372.8e8, 418.6e8, 464.8e8, 511.6e8, 558.9e8, 606.8e8, 655.2e8, 704.1e8,
+     753.5e8, 803.6e8, 854.3e8, 905.6e8, 957.6e8, 1010.3e8, 1063.8e8, 1118.2e8, 1173.4e8,
+     1229.7e8, 1269.7e8, 1287.0e8, 1345.6e8, 1357.5e8}};
+
+__device__ double ref_pressure(const des_params *P, double z)   // matprops.cxx:189-201
+{
+    double depth = -z;
+    if (P->ref_pressure_option == 0) return P->rho0[0] * P->gravity * depth;
+    if (P->ref_pressure_option == 1 || P->ref_pressure_option == 2) {
+        const double *tab = c_prem_p[P->ref_pressure_option - 1];
+        if (depth <= 0) return 0;
+        int n;
+        for (n = 1; n < 46; n++)
+            if (depth <= c_prem_depth[n]) break;
+        return tab[n - 1] + (tab[n] - tab[n - 1]) * (depth - c_prem_depth[n - 1])
+            / (c_prem_depth[n] - c_prem_depth[n - 1]);
+    }
+    return 0;
+}
+
+// traction of one loaded boundary facet (apply_stress_bcs, bc.cxx:502-556):
+// ftrac[k] = p * normal / NODES_PER_FACET
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevState s, const int *facets,
+                                                                          int nf)
+{
+    int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nf) return;
+    int k = facets[t];
+    const des_params *P = s.P;
+    int i = s.bf_bdry[k], e = s.bf_elem[k], f = s.bf_face[k];
+    constexpr int NPE = Dim<ND>::NPE;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    static constexpr int NOF3[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+    static constexpr int NOF2[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+    double normal[ND], zcenter;
+    if constexpr (ND == 3) {
+        int n0 = c[NOF3[f][0]], n1 = c[NOF3[f][1]], n2 = c[NOF3[f][2]];
+        double v01[3], v02[3];
+        for (int d = 0; d < 3; ++d) {
+            v01[d] = s.coord[d * s.N + n1] - s.coord[d * s.N + n0];
+            v02[d] = s.coord[d * s.N + n2] - s.coord[d * s.N + n0];
+        }
+        normal[0] = (v01[1] * v02[2] - v01[2] * v02[1]) / 2;
+        normal[1] = (v01[2] * v02[0] - v01[0] * v02[2]) / 2;
+        normal[2] = (v01[0] * v02[1] - v01[1] * v02[0]) / 2;
+        zcenter = (s.coord[2 * s.N + n0] + s.coord[2 * s.N + n1] + s.coord[2 * s.N + n2]) / 3;
+    } else {
+        int n0 = c[NOF2[f][0]], n1 = c[NOF2[f][1]];
+        double v01[2];
+        for (int d = 0; d < 2; ++d) v01[d] = s.coord[d * s.N + n1] - s.coord[d * s.N + n0];
+        normal[0] = v01[1];
+        normal[1] = -v01[0];
+        zcenter = (s.coord[s.N + n0] + s.coord[s.N + n1]) / 2;
+    }
+    double p;
+    if (i == 4 && P->has_winkler_foundation) {
+        p = s.sc->compensation_pressure -
+            (mat_rho(s, e, elem_mean<ND>(s.T, c)) + P->winkler_delta_rho) * P->gravity * (zcenter + P->zlength);
+    } else if (i == 5 && P->has_water_loading) {
+        const double sea_level = 0;
+        p = 0;
+        if (zcenter < sea_level) {
+            const double sea_water_density = 1030;
+            p = sea_water_density * P->gravity * (sea_level - zcenter);
+        }
+    } else {
+        p = ref_pressure(P, zcenter);
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s.ftrac[k * ND + d] = p * normal[d] / ND;
+}
+
+// node part of update_force: gather, stress BCs, elastic foundation,
+// damping (fields.cxx:291-340, bc.cxx:493-566, fields.cxx:235-288)
+template <int ND> __device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND],
+                                                           const double v[ND])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    const des_params *P = s.P;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) f[d] = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
+        const double *c = s.ework + (long)s.csr_ent[p] * ND;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) f[d] -= c[d];
+    }
+    if (s.ff_off) {
+        for (int p = s.ff_off[n]; p < s.ff_off[n + 1]; ++p) {
+            const double *t = s.ftrac + (long)s.ff_ent[p] * ND;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) f[d] -= t[d];
+        }
+    }
+    if (P->gravity != 0 && P->has_elastic_foundation && (s.bcflag[n] & BOUNDZ0)) {
+        f[ND - 1] -= P->elastic_foundation_constant *
+            (s.coord[(ND - 1) * s.N + n] - s.coord0[(ND - 1) * s.N + n]);
+    }
+    if (P->is_quasi_static) {
+        const double small_vel = 1e-13;
+        const double df = P->damping_factor;
+        switch (P->damping_option) {
+        case 1:
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+                if (fabs(v[d]) > small_vel) f[d] -= df * copysign(f[d], v[d]);
+            break;
+        case 2:
+#pragma unroll
+            for (int d = 0; d < ND; ++d) f[d] -= df * f[d];
+            break;
+        case 3:
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                if ((f[d] < 0) == (v[d] < 0)) f[d] -= df * f[d];
+                else f[d] += (1 - df) * f[d];
+            }
+            break;
+        default:
+            break;
+        }
+    }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_node(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    double f[ND], v[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] = s.vel[d * s.N + n];
+    node_force<ND>(s, n, f, v);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_velocity(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    const double dt = s.sc->dt, m = s.mass[n];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s.vel[d * s.N + n] += dt * s.force[d * s.N + n] / m;
+}
+
+// apply_vbcs for one node (bc.cxx:179-490); returns false if an edge vector
+// the reference's map lacks is needed
+template <int ND> __device__ __forceinline__ bool vbc_node(const DevState &s, unsigned flag, double v[ND])
+{
+    if (!(flag & BOUND_ANY)) return true;
+    const int *T = s.P->vbc_types;
+    const double *V = s.P->vbc_values;
+    bool ok = true;
+    if (flag & BOUNDX0) {
+        switch (T[0]) {
+        case 1: v[0] = V[0]; break;
+        case 2: v[1] = 0; if constexpr (ND == 3) v[2] = 0; break;
+        case 3: v[0] = V[0]; v[1] = 0; if constexpr (ND == 3) v[2] = 0; break;
+        case 4: if constexpr (ND == 3) { v[1] = V[0]; v[2] = 0; } break;
+        case 5: if constexpr (ND == 3) { v[0] = 0; v[1] = V[0]; v[2] = 0; } break;
+        case 7: if constexpr (ND == 3) { v[0] = V[0]; v[1] = 0; } break;
+        }
+    }
+    if (flag & BOUNDX1) {
+        switch (T[1]) {
+        case 1: v[0] = V[1]; break;
+        case 2: v[1] = 0; if constexpr (ND == 3) v[2] = 0; break;
+        case 3: v[0] = V[1]; v[1] = 0; if constexpr (ND == 3) v[2] = 0; break;
+        case 4: if constexpr (ND == 3) { v[1] = V[1]; v[2] = 0; } break;
+        case 5: if constexpr (ND == 3) { v[0] = 0; v[1] = V[1]; v[2] = 0; } break;
+        case 7: if constexpr (ND == 3) { v[0] = V[1]; v[1] = 0; } break;
+        }
+    }
+    if constexpr (ND == 3) {
+        if (flag & BOUNDY0) {
+            switch (T[2]) {
+            case 1: v[1] = V[2]; break;
+            case 2: v[0] = 0; v[2] = 0; break;
+            case 3: v[0] = 0; v[1] = V[2]; v[2] = 0; break;
+            case 4: v[0] = V[2]; v[2] = 0; break;
+            case 5: v[0] = V[2]; v[1] = 0; v[2] = 0; break;
+            case 7: v[0] = 0; v[1] = V[2]; break;
+            }
+        }
+        if (flag & BOUNDY1) {
+            switch (T[3]) {
+            case 1: v[1] = V[3]; break;
+            case 2: v[0] = 0; v[2] = 0; break;
+            case 3: v[0] = V[3]; v[1] = 0; v[2] = 0; break;   // as bc.cxx:325-329
+            case 4: v[0] = V[3]; v[2] = 0; break;
+            case 5: v[0] = V[3]; v[1] = 0; v[2] = 0; break;
+            case 7: v[0] = 0; v[1] = V[3]; break;
+            }
+        }
+    }
+    for (int ib = IBOUNDN0; ib <= IBOUNDN3; ib++) {
+        if (!(flag & (1u << ib))) continue;
+        const double *nv = s.bnormals + ib * ND;
+        double fac = 0;
+        switch (T[ib]) {
+        case 1:
+        case 11: {
+            const int nh = (T[ib] == 1) ? ND : ND - 1;   // type 11: horizontal components only
+            if (T[ib] == 11) fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
+            const double target = (T[ib] == 1) ? V[ib] : V[ib] * fac;
+            if (flag == (1u << ib)) {
+                double vn = 0;
+                for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
+                for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
+            } else {
+                for (int ic = 0; ic < ib; ic++) {
+                    if (!(flag & (1u << ic))) continue;
+                    if (T[ic] == 0) {
+                        double vn = 0;
+                        for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
+                        for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
+                    } else if (T[ic] == 1) {
+                        if (!s.edge_valid[ic * DES_NBDRY + ib]) { ok = false; continue; }
+                        const double *edge = s.edge_vectors + (ic * DES_NBDRY + ib) * ND;
+                        double ve = 0;
+                        for (int d = 0; d < ND; d++) ve += v[d] * edge[d];
+                        for (int d = 0; d < ND; d++) v[d] = ve * edge[d];
+                    }
+                }
+            }
+            break;
+        }
+        case 3:
+            for (int d = 0; d < ND; d++) v[d] = V[ib] * nv[d];
+            break;
+        case 13:
+            fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
+            for (int d = 0; d < ND - 1; d++) v[d] = V[ib] * fac * nv[d];
+            v[ND - 1] = 0;
+            break;
+        }
+    }
+    if (T[4] == 0 && T[5] == 0) return ok;
+    if (flag & BOUNDZ0) {
+        switch (T[4]) {
+        case 1: v[ND - 1] = V[4]; break;
+        case 2: v[0] = 0; if constexpr (ND == 3) v[1] = 0; break;
+        case 3: v[0] = 0; if constexpr (ND == 3) v[1] = 0; v[ND - 1] = V[4]; break;
+        }
+    }
+    if (flag & BOUNDZ1) {
+        switch (T[5]) {
+        case 1: v[ND - 1] = V[5]; break;
+        case 2: v[0] = 0; if constexpr (ND == 3) v[1] = 0; break;
+        case 3: v[0] = 0; if constexpr (ND == 3) v[1] = 0; v[ND - 1] = V[5]; break;
+        }
+    }
+    return ok;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_apply_vbcs(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    unsigned flag = s.bcflag[n];
+    if (!(flag & BOUND_ANY)) return;
+    double v[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] = s.vel[d * s.N + n];
+    if (!vbc_node<ND>(s, flag, v)) set_error(s.sc, DES_ERR_ASSERT);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s.vel[d * s.N + n] = v[d];
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_coordinate(DevState s)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    const double dt = s.sc->dt;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s.coord[d * s.N + n] += s.vel[d * s.N + n] * dt;
+}
+
+// rotate_stress (fields.cxx:411-483) with stored shape functions
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_rotate(DevState s, int use_dt_prev)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double v[NPE][ND], sh[ND][NPE], w[3], st[NSTR], es[NSTR];
+    gather_vec<ND>(s.vel, s.N, c, v);
+    load_shp<ND>(s, e, sh);
+    spin<ND>(v, sh, w);
+    const double dt = use_dt_prev ? s.sc->dt_prev : s.sc->dt;
+    load_tensor<ND>(s.stress, s.E, e, st);
+    load_tensor<ND>(s.strain, s.E, e, es);
+    jaumann<ND>(st, dt, w);
+    jaumann<ND>(es, dt, w);
+    store_tensor<ND>(s.stress, s.E, e, st);
+    store_tensor<ND>(s.strain, s.E, e, es);
+}
+
+// ---- compute_dt (geometry.cxx:239-310): block minima, then one block
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial(DevState s)
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    const des_params *P = s.P;
+    double dtm = DBL_MAX, dtd = DBL_MAX, minl = DBL_MAX;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) {
+        int c[NPE];
+        load_conn<ND>(s, e, c);
+        double x[NPE][ND];
+        gather_vec<ND>(s.coord, s.N, c, x);
+        double vol = elem_volume<ND>(x);
+        double minh;
+        if constexpr (ND == 3) {
+            double maxa = dmax(dmax(triangle_area<3>(x[0], x[1], x[2]), triangle_area<3>(x[0], x[1], x[3])),
+                               dmax(triangle_area<3>(x[2], x[3], x[0]), triangle_area<3>(x[2], x[3], x[1])));
+            minh = 3 * vol / maxa;
+        } else {
+            double maxl = sqrt(dmax(dmax(dist2<2>(x[0], x[1]), dist2<2>(x[1], x[2])), dist2<2>(x[0], x[2])));
+            minh = 2 * vol / maxl;
+        }
+        dtm = dmin(dtm, 0.5 * P->visc_min / (1e-40 + harmonic_mean(s, P->shear_modulus, e)));
+        if (P->has_thermal_diffusion) dtd = dmin(dtd, 0.5 * minh * minh / P->therm_diff_max);
+        minl = dmin(minl, minh);
+    }
+    __shared__ double sm[3][BLOCK];
+    sm[0][threadIdx.x] = dtm;
+    sm[1][threadIdx.x] = dtd;
+    sm[2][threadIdx.x] = minl;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int j = 0; j < 3; ++j) sm[j][threadIdx.x] = dmin(sm[j][threadIdx.x], sm[j][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) s.red[blockIdx.x * 3 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb, double bulkm0, double rho0)
+{
+    const des_params *P = s.P;
+    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int b = threadIdx.x; b < nb; b += BLOCK)
+        for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
+    __shared__ double sm[3][BLOCK];
+    for (int j = 0; j < 3; ++j) sm[j][threadIdx.x] = v[j];
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int j = 0; j < 3; ++j) sm[j][threadIdx.x] = dmin(sm[j][threadIdx.x], sm[j][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double dt;
+        if (P->fixed_dt != 0) {
+            dt = P->fixed_dt;
+        } else {
+            double dt_maxwell = sm[0][0], dt_diffusion = sm[1][0], minl = sm[2][0];
+            double dt_advection = 0.5 * minl / P->max_vbc_val;
+            double dt_elastic = P->is_quasi_static ?
+                0.5 * minl / (P->max_vbc_val * P->inertial_scaling) :
+                0.5 * minl / sqrt(bulkm0 / rho0);
+            dt = dmin(dmin(dt_elastic, dt_maxwell), dmin(dt_advection, dt_diffusion)) * P->dt_fraction;
+            if (!(dt > 0)) set_error(s.sc, DES_ERR_RUNTIME);
+        }
+        s.sc->dt = dt;
+    }
+}
+
+// rho(0) and bulkm(0) for the dynamic elastic dt (geometry.cxx:297)
+template <int ND> __global__ void k_elem0_props(DevState s, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int c[Dim<ND>::NPE];
+    load_conn<ND>(s, 0, c);
+    out[0] = harmonic_mean(s, s.P->bulk_modulus, 0);
+    out[1] = mat_rho(s, 0, elem_mean<ND>(s.T, c));
+}
+
+// ---- worst_elem_quality (geometry.cxx:458-512)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial(DevState s)
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    double q = DBL_MAX;
+    int qi = 0x7fffffff;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) {
+        int c[NPE];
+        load_conn<ND>(s, e, c);
+        double x[NPE][ND];
+        gather_vec<ND>(s.coord, s.N, c, x);
+        double vol = s.volume[e], quality;
+        if constexpr (ND == 3) {
+            double normalization_factor = 216 * sqrt(3.0);
+            double area_sum = (triangle_area<3>(x[0], x[1], x[2]) + triangle_area<3>(x[0], x[1], x[3]) +
+                               triangle_area<3>(x[2], x[3], x[0]) + triangle_area<3>(x[2], x[3], x[1]));
+            quality = normalization_factor * vol * vol / (area_sum * area_sum * area_sum);
+        } else {
+            double normalization_factor = 4 * sqrt(3.0);
+            double dist2_sum = dist2<2>(x[0], x[1]) + dist2<2>(x[1], x[2]) + dist2<2>(x[0], x[2]);
+            quality = normalization_factor * vol / dist2_sum;
+        }
+        if (quality < q) { q = quality; qi = e; }
+    }
+    __shared__ double sq[BLOCK];
+    __shared__ int si[BLOCK];
+    sq[threadIdx.x] = q;
+    si[threadIdx.x] = qi;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double q2 = sq[threadIdx.x + w];
+            int i2 = si[threadIdx.x + w];
+            if (q2 < sq[threadIdx.x] || (q2 == sq[threadIdx.x] && i2 < si[threadIdx.x])) {
+                sq[threadIdx.x] = q2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s.red[blockIdx.x] = sq[0];
+        s.redi[blockIdx.x] = si[0];
+    }
+}
+
+// ============================================================ fused step
+//
+// Pass A (elements): volume, shape functions, thermal contributions,
+//   volume-weighted dilatation, element mass and thermal mass.
+// Pass B (nodes):    temperature update, volume_n, mass, dvoldt.
+// Pass C (elements): deferred Jaumann rotation, strain rate, anti-locking,
+//   rheology.
+// Pass D (nodes):    volume-weighted nodal pressure change.
+// Pass E (elements): nodal-mixed pressure correction, force contributions.
+// Pass F (nodes):    force gather and BCs, damping, velocity, velocity BCs,
+//   coordinates.
+
+struct FusedFlags {
+    int use_stored_mass;   // first step after a sync: mass/tmass/volume_n from compute_mass
+    int store_aux;         // last step before a sync: keep strain_rate, edvoldt, force
+};
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    const des_params *P = s.P;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND], sh[ND][NPE], v[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    double vol = elem_volume<ND>(x);
+    shape_fn<ND>(x, vol, sh);
+    s.volume[e] = vol;
+    double Tn[NPE];
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) Tn[i] = s.T[c[i]];
+    if (P->has_thermal_diffusion) {
+        double out[NPE];
+        temperature_contrib<ND>(s, e, vol, sh, Tn, out);
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) s.ework[e * NPE + i] = out[i];
+    }
+    gather_vec<ND>(s.vel, s.N, c, v);
+    double sr[NSTR];
+    strain_rate<ND>(v, sh, sr);
+    s.escal[0 * E + e] = trace<ND>(sr) * vol;
+    if (!fl.use_stored_mass) {
+        double Tm = 0;
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) Tm += Tn[i];
+        Tm /= NPE;
+        double m, tm;
+        mass_of<ND>(s, e, vol, Tm, m, tm);
+        s.escal[1 * E + e] = m;
+        s.escal[2 * E + e] = tm;
+    }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    const long E = s.E;
+    const int thermal = s.P->has_thermal_diffusion;
+    double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
+        const int ent = s.csr_ent[p];
+        const int e = ent / NPE;
+        if (thermal) tdot += s.ework[ent];
+        dj += s.escal[e];
+        if (!fl.use_stored_mass) {
+            vn += s.volume[e];
+            m += s.escal[1 * E + e];
+            if (thermal) tm += s.escal[2 * E + e];
+        }
+    }
+    if (fl.use_stored_mass) {
+        vn = s.volume_n[n];
+        m = s.mass[n];
+        tm = s.tmass[n];
+    } else {
+        s.volume_n[n] = vn;
+        s.mass[n] = m;
+        s.tmass[n] = tm;
+    }
+    if (thermal) s.T[n] = temperature_new(s, n, s.T[n], tdot, s.sc->dt, tm);
+    s.ntmp[n] = dj / vn;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c(DevState s, FusedFlags fl)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND], sh[ND][NPE], v[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    const double vol = s.volume[e];
+    shape_fn<ND>(x, vol, sh);
+    gather_vec<ND>(s.vel, s.N, c, v);
+    double st[NSTR], es[NSTR], edot[NSTR];
+    load_tensor<ND>(s.stress, E, e, st);
+    load_tensor<ND>(s.strain, E, e, es);
+    if (s.sc->rot_pending) {
+        double w[3];
+        spin<ND>(v, sh, w);
+        const double dtp = s.sc->dt_prev;
+        jaumann<ND>(st, dtp, w);
+        jaumann<ND>(es, dtp, w);
+    }
+    strain_rate<ND>(v, sh, edot);
+    double edv = 0;
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) edv += s.ntmp[c[i]];
+    edv = edv / NPE;
+    double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
+    double Tm = elem_mean<ND>(s.T, c);
+    bool ok = stress_update<ND>(s, e, s.sc->dt, edv, Tm, vol, s.volume_old[e], st, syy, es, edot,
+                                pl, dpl, dp);
+    if (!ok) set_error(s.sc, DES_ERR_INPUT);
+    store_tensor<ND>(s.stress, E, e, st);
+    store_tensor<ND>(s.strain, E, e, es);
+    if (s.P->is_plane_strain) s.stressyy[e] = syy;
+    s.plstrain[e] = pl;
+    s.delta_plstrain[e] = dpl;
+    s.dpressure[e] = dp;
+    if (fl.store_aux) {
+        store_tensor<ND>(s.strain_rate, E, e, edot);
+        s.edvoldt[e] = edv;
+    }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double st[NSTR];
+    load_tensor<ND>(s.stress, E, e, st);
+    nmd_correct<ND>(s, e, c, st);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s.stress[i * E + e] = st[i];
+    double x[NPE][ND], sh[ND][NPE], out[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    const double vol = s.volume[e];
+    shape_fn<ND>(x, vol, sh);
+    force_contrib<ND>(s, e, vol, sh, st, elem_mean<ND>(s.T, c), out);
+    store_force_contrib<ND>(s, e, out);
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, FusedFlags fl)
+{
+    int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    const double dt = s.sc->dt;
+    double f[ND], v[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] = s.vel[d * s.N + n];
+    node_force<ND>(s, n, f, v);
+    if (fl.store_aux) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
+    }
+    const double m = s.mass[n];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] += dt * f[d] / m;
+    unsigned flag = s.bcflag[n];
+    if (flag & BOUND_ANY)
+        if (!vbc_node<ND>(s, flag, v)) set_error(s.sc, DES_ERR_ASSERT);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        s.vel[d * s.N + n] = v[d];
+        s.coord[d * s.N + n] += v[d] * dt;
+    }
+}
+
+// step bookkeeping after pass F: var.time, var.steps, the rotation to defer
+__global__ void k_step_end(DevState s, int rotate)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DevScalars *sc = s.sc;
+    sc->time += sc->dt;
+    sc->steps += 1;
+    sc->dt_prev = sc->dt;
+    sc->rot_pending = rotate;
+}
+
+__global__ void k_clear_rotation(DevState s)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    s.sc->rot_pending = 0;
+}
+
+__global__ void k_step_begin(DevState s)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    s.sc->time += s.sc->dt;
+    s.sc->steps += 1;
+}
+
+}  // namespace
+
+// ================================================================ host side
+
+struct des_ctx {
+    des_params P;
+    int nd, npe, nstr, N, E, nbf;
+    hipStream_t stream = nullptr;
+    DevState s{};
+    std::vector<void *> allocs;
+    std::string err;
+    double *vol_buf[2] = {nullptr, nullptr};
+    int vol_cur = 0;               // s.volume == vol_buf[vol_cur]
+    int *active_facets = nullptr;  // facets whose traction enters the force
+    int nactive = 0;
+    double *elem0 = nullptr;       // bulkm(0), rho(0)
+    int steps = 0;                 // host mirror of var.steps
+    bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
+};
+
+namespace {
+
+int fail(des_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(ctx, DES_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int dalloc(des_ctx *ctx, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    HIPCHK(hipMalloc(p, bytes));
+    ctx->allocs.push_back(*p);
+    HIPCHK(hipMemsetAsync(*p, 0, bytes, ctx->stream));
+    return DES_OK;
+}
+
+template <typename T> int dalloc_n(des_ctx *ctx, T **p, size_t n)
+{
+    return dalloc(ctx, reinterpret_cast<void **>(p), n * sizeof(T));
+}
+
+int check_launch(des_ctx *ctx)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, DES_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return DES_OK;
+}
+
+#define LAUNCH_ND(kern, grid, ...)                                                        \
+    do {                                                                                  \
+        if (ctx->nd == 3) hipLaunchKernelGGL(kern<3>, dim3(grid), dim3(BLOCK), 0, ctx->stream, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern<2>, dim3(grid), dim3(BLOCK), 0, ctx->stream, __VA_ARGS__); \
+        int rc_ = check_launch(ctx);                                                      \
+        if (rc_) return rc_;                                                              \
+    } while (0)
+
+bool facet_loaded(const des_params &P, int i)
+{
+    if (P.gravity == 0) return false;
+    int t = P.vbc_types[i];
+    if (t != 0 && t != 2 && t != 4) return false;
+    if (i == 4 && !P.has_winkler_foundation) return false;
+    if (i == 5 && !P.has_water_loading) return false;
+    return true;
+}
+
+// host AoS [n][k] <-> device SoA [k][n]
+int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k)
+{
+    if (!src) return DES_OK;
+    std::vector<double> tmp((size_t)n * k);
+    for (long i = 0; i < n; ++i)
+        for (int j = 0; j < k; ++j) tmp[(size_t)j * n + i] = src[(size_t)i * k + j];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DES_OK;
+}
+
+int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k)
+{
+    if (!dst) return DES_OK;
+    std::vector<double> tmp((size_t)n * k);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (long i = 0; i < n; ++i)
+        for (int j = 0; j < k; ++j) dst[(size_t)i * k + j] = tmp[(size_t)j * n + i];
+    return DES_OK;
+}
+
+int read_scalars(des_ctx *ctx, DevScalars &h)
+{
+    HIPCHK(hipMemcpyAsync(&h, ctx->s.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DES_OK;
+}
+
+void swap_volume(des_ctx *ctx)
+{
+    ctx->vol_cur ^= 1;
+    ctx->s.volume = ctx->vol_buf[ctx->vol_cur];
+    ctx->s.volume_old = ctx->vol_buf[ctx->vol_cur ^ 1];
+}
+
+int launch_compute_dt(des_ctx *ctx)
+{
+    if (ctx->P.fixed_dt == 0) LAUNCH_ND(k_dt_partial, RED_BLOCKS, ctx->s);
+    double bulkm0 = 0, rho0 = 1;
+    if (!ctx->P.is_quasi_static && ctx->P.fixed_dt == 0) {
+        LAUNCH_ND(k_elem0_props, 1, ctx->s, ctx->elem0);
+        double h[2];
+        HIPCHK(hipMemcpyAsync(h, ctx->elem0, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        bulkm0 = h[0];
+        rho0 = h[1];
+    }
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLOCK), 0, ctx->stream, ctx->s, RED_BLOCKS, bulkm0, rho0);
+    return check_launch(ctx);
+}
+
+int launch_mass(des_ctx *ctx)
+{
+    LAUNCH_ND(k_mass_elem, nblocks(ctx->E), ctx->s);
+    LAUNCH_ND(k_mass_node, nblocks(ctx->N), ctx->s);
+    return DES_OK;
+}
+
+int launch_update_mesh(des_ctx *ctx)
+{
+    LAUNCH_ND(k_update_coordinate, nblocks(ctx->N), ctx->s);
+    swap_volume(ctx);
+    LAUNCH_ND(k_volume, nblocks(ctx->E), ctx->s);
+    int rc = launch_mass(ctx);
+    if (rc) return rc;
+    LAUNCH_ND(k_shape_fn, nblocks(ctx->E), ctx->s);
+    return DES_OK;
+}
+
+int launch_force(des_ctx *ctx)
+{
+    LAUNCH_ND(k_force_elem, nblocks(ctx->E), ctx->s);
+    if (ctx->nactive) LAUNCH_ND(k_facet_traction, nblocks(ctx->nactive), ctx->s, ctx->active_facets, ctx->nactive);
+    LAUNCH_ND(k_force_node, nblocks(ctx->N), ctx->s);
+    return DES_OK;
+}
+
+// bring the device state back to the reference's Variables after des_run
+int flush_deferred(des_ctx *ctx)
+{
+    if (!ctx->closing_pending) return DES_OK;
+    // the tail of the last step: update_mesh after update_coordinate, then
+    // rotate_stress with that step's dt (dynearthsol.cxx:226-231, 346-347)
+    swap_volume(ctx);
+    LAUNCH_ND(k_volume, nblocks(ctx->E), ctx->s);
+    int rc = launch_mass(ctx);
+    if (rc) return rc;
+    LAUNCH_ND(k_shape_fn, nblocks(ctx->E), ctx->s);
+    if (ctx->P.rheol_type & DES_RH_ELASTIC) LAUNCH_ND(k_rotate, nblocks(ctx->E), ctx->s, 1);
+    hipLaunchKernelGGL(k_clear_rotation, dim3(1), dim3(1), 0, ctx->stream, ctx->s);
+    ctx->closing_pending = false;
+    return check_launch(ctx);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+
+extern "C" {
+
+const char *des_last_error(const des_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
+{
+    if (!p || !m || !out) return DES_ERR_INPUT;
+    *out = nullptr;
+    if (p->ndims != 2 && p->ndims != 3) return DES_ERR_INPUT;
+    if (p->nmat < 1 || p->nmat > DES_MAX_MAT) return DES_ERR_INPUT;
+    if (m->nnode <= 0 || m->nelem <= 0 || !m->connectivity || !m->coord || !m->bcflag) return DES_ERR_MESH;
+    des_ctx *ctx = new des_ctx();
+    ctx->P = *p;
+    const int nd = ctx->nd = p->ndims;
+    const int npe = ctx->npe = nd + 1;
+    ctx->nstr = nd == 3 ? 6 : 3;
+    const int N = ctx->N = m->nnode, E = ctx->E = m->nelem;
+    ctx->nbf = m->nbfacet;
+    // validate the mesh on the host: every index the kernels will follow
+    for (long k = 0; k < (long)E * npe; ++k)
+        if (m->connectivity[k] < 0 || m->connectivity[k] >= N) {
+            delete ctx;
+            return DES_ERR_MESH;
+        }
+    for (int k = 0; k < m->nbfacet; ++k)
+        if (m->bfacet_bdry[k] < 0 || m->bfacet_bdry[k] >= DES_NBDRY || m->bfacet_elem[k] < 0 ||
+            m->bfacet_elem[k] >= E || m->bfacet_face[k] < 0 || m->bfacet_face[k] > nd) {
+            delete ctx;
+            return DES_ERR_MESH;
+        }
+    if (!m->elemmarkers && p->nmat != 1) {
+        delete ctx;
+        return DES_ERR_INPUT;
+    }
+    {
+        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            int rc = fail(ctx, DES_ERR_HIP, hipGetErrorString(e));
+            *out = ctx;
+            return rc;
+        }
+    }
+    *out = ctx;
+    DevState &s = ctx->s;
+    s.N = N;
+    s.E = E;
+    s.nbf = m->nbfacet;
+    s.mpe = m->markers_per_element;
+    int rc;
+#define ALLOC(ptr, n) do { if ((rc = dalloc_n(ctx, &(ptr), (size_t)(n)))) return rc; } while (0)
+    des_params *dP;
+    ALLOC(dP, 1);
+    HIPCHK(hipMemcpyAsync(dP, p, sizeof(des_params), hipMemcpyHostToDevice, ctx->stream));
+    s.P = dP;
+    int *conn;
+    ALLOC(conn, (size_t)E * npe);
+    HIPCHK(hipMemcpyAsync(conn, m->connectivity, sizeof(int) * E * npe, hipMemcpyHostToDevice, ctx->stream));
+    s.conn = conn;
+    unsigned *bcflag;
+    ALLOC(bcflag, N);
+    HIPCHK(hipMemcpyAsync(bcflag, m->bcflag, sizeof(unsigned) * N, hipMemcpyHostToDevice, ctx->stream));
+    s.bcflag = bcflag;
+    if (m->elemmarkers) {
+        int *em;
+        ALLOC(em, (size_t)E * p->nmat);
+        HIPCHK(hipMemcpyAsync(em, m->elemmarkers, sizeof(int) * E * p->nmat, hipMemcpyHostToDevice, ctx->stream));
+        s.elemmarkers = em;
+    }
+    // create_support (mesh.cxx:1707-1721) as a CSR, ascending elements per node
+    std::vector<int> off(N + 1, 0), ent((size_t)E * npe);
+    for (long k = 0; k < (long)E * npe; ++k) off[m->connectivity[k] + 1]++;
+    for (int n = 0; n < N; ++n) off[n + 1] += off[n];
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int e = 0; e < E; ++e)
+            for (int i = 0; i < npe; ++i) ent[fill[m->connectivity[(size_t)e * npe + i]]++] = e * npe + i;
+    }
+    int *d_off, *d_ent;
+    ALLOC(d_off, N + 1);
+    ALLOC(d_ent, (size_t)E * npe);
+    HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_ent, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice, ctx->stream));
+    s.csr_off = d_off;
+    s.csr_ent = d_ent;
+    // boundary facets; the loaded ones get a per-node list in reference order
+    if (m->nbfacet > 0) {
+        int *b, *e, *f;
+        ALLOC(b, m->nbfacet);
+        ALLOC(e, m->nbfacet);
+        ALLOC(f, m->nbfacet);
+        HIPCHK(hipMemcpyAsync(b, m->bfacet_bdry, sizeof(int) * m->nbfacet, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(e, m->bfacet_elem, sizeof(int) * m->nbfacet, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(f, m->bfacet_face, sizeof(int) * m->nbfacet, hipMemcpyHostToDevice, ctx->stream));
+        s.bf_bdry = b;
+        s.bf_elem = e;
+        s.bf_face = f;
+        static const int NOF3[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+        static const int NOF2[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+        std::vector<int> act;
+        for (int i = 0; i < DES_NBDRY; ++i) {
+            if (!facet_loaded(*p, i)) continue;
+            for (int k = 0; k < m->nbfacet; ++k)
+                if (m->bfacet_bdry[k] == i) act.push_back(k);
+        }
+        if (!act.empty()) {
+            std::vector<int> foff(N + 1, 0);
+            auto facet_node = [&](int k, int j) {
+                int el = m->bfacet_elem[k], fc = m->bfacet_face[k];
+                int loc = nd == 3 ? NOF3[fc][j] : NOF2[fc][j];
+                return m->connectivity[(size_t)el * npe + loc];
+            };
+            for (int k : act)
+                for (int j = 0; j < nd; ++j) foff[facet_node(k, j) + 1]++;
+            for (int n = 0; n < N; ++n) foff[n + 1] += foff[n];
+            std::vector<int> fent(foff[N]), fill(foff.begin(), foff.end() - 1);
+            for (int k : act)
+                for (int j = 0; j < nd; ++j) fent[fill[facet_node(k, j)]++] = k;
+            int *d_foff, *d_fent, *d_act;
+            ALLOC(d_foff, N + 1);
+            ALLOC(d_fent, fent.size());
+            ALLOC(d_act, act.size());
+            HIPCHK(hipMemcpyAsync(d_foff, foff.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(d_fent, fent.data(), sizeof(int) * fent.size(), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(d_act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, ctx->stream));
+            s.ff_off = d_foff;
+            s.ff_ent = d_fent;
+            ctx->active_facets = d_act;
+            ctx->nactive = (int)act.size();
+            ALLOC(s.ftrac, (size_t)m->nbfacet * nd);
+        }
+    }
+    double *bn, *ev;
+    int *evv;
+    ALLOC(bn, DES_NBDRY * nd);
+    ALLOC(ev, DES_NBDRY * DES_NBDRY * nd);
+    ALLOC(evv, DES_NBDRY * DES_NBDRY);
+    if (m->bnormals)
+        HIPCHK(hipMemcpyAsync(bn, m->bnormals, sizeof(double) * DES_NBDRY * nd, hipMemcpyHostToDevice, ctx->stream));
+    if (m->edge_vectors)
+        HIPCHK(hipMemcpyAsync(ev, m->edge_vectors, sizeof(double) * DES_NBDRY * DES_NBDRY * nd, hipMemcpyHostToDevice, ctx->stream));
+    if (m->edge_valid)
+        HIPCHK(hipMemcpyAsync(evv, m->edge_valid, sizeof(int) * DES_NBDRY * DES_NBDRY, hipMemcpyHostToDevice, ctx->stream));
+    s.bnormals = bn;
+    s.edge_vectors = ev;
+    s.edge_valid = evv;
+    // fields (zero, as allocate_variables leaves them)
+    ALLOC(s.coord, (size_t)N * nd);
+    ALLOC(s.coord0, (size_t)N * nd);
+    ALLOC(s.vel, (size_t)N * nd);
+    ALLOC(s.force, (size_t)N * nd);
+    ALLOC(s.T, N);
+    ALLOC(s.mass, N);
+    ALLOC(s.tmass, N);
+    ALLOC(s.volume_n, N);
+    ALLOC(s.ntmp, N);
+    ALLOC(ctx->vol_buf[0], E);
+    ALLOC(ctx->vol_buf[1], E);
+    s.volume = ctx->vol_buf[0];
+    s.volume_old = ctx->vol_buf[1];
+    ALLOC(s.edvoldt, E);
+    ALLOC(s.plstrain, E);
+    ALLOC(s.delta_plstrain, E);
+    ALLOC(s.dpressure, E);
+    ALLOC(s.stressyy, E);
+    ALLOC(s.strain_rate, (size_t)E * ctx->nstr);
+    ALLOC(s.strain, (size_t)E * ctx->nstr);
+    ALLOC(s.stress, (size_t)E * ctx->nstr);
+    ALLOC(s.shp, (size_t)E * npe * nd);
+    ALLOC(s.ework, (size_t)E * npe * nd);
+    ALLOC(s.escal, (size_t)E * 4);
+    ALLOC(s.sc, 1);
+    ALLOC(s.red, RED_BLOCKS * 3);
+    ALLOC(s.redi, RED_BLOCKS);
+    ALLOC(ctx->elem0, 2);
+#undef ALLOC
+    // initial coordinates, also coord0 (dynearthsol.cxx:82-84)
+    if ((rc = upload_soa(ctx, s.coord, m->coord, N, nd))) return rc;
+    if ((rc = upload_soa(ctx, s.coord0, m->coord, N, nd))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DES_OK;
+}
+
+void des_destroy(des_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+static int field_opt(des_ctx *ctx, double **slot, const double *src, long n)
+{
+    if (!src) return DES_OK;
+    if (!*slot) {
+        int rc = dalloc_n(ctx, slot, (size_t)n);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(*slot, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    return DES_OK;
+}
+
+int des_upload_fields(des_ctx *ctx, const des_fields *f)
+{
+    if (!ctx || !f) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    DevState &s = ctx->s;
+    const long N = ctx->N, E = ctx->E;
+    const int nd = ctx->nd, npe = ctx->npe, nstr = ctx->nstr;
+    struct { double *dst; const double *src; long n; int k; } tab[] = {
+        {s.coord, f->coord, N, nd}, {s.coord0, f->coord0, N, nd}, {s.vel, f->vel, N, nd},
+        {s.force, f->force, N, nd}, {s.T, f->temperature, N, 1}, {s.mass, f->mass, N, 1},
+        {s.tmass, f->tmass, N, 1}, {s.volume_n, f->volume_n, N, 1}, {s.ntmp, f->ntmp, N, 1},
+        {s.volume, f->volume, E, 1}, {s.volume_old, f->volume_old, E, 1},
+        {s.edvoldt, f->edvoldt, E, 1}, {s.plstrain, f->plstrain, E, 1},
+        {s.delta_plstrain, f->delta_plstrain, E, 1}, {s.dpressure, f->dpressure, E, 1},
+        {s.stressyy, f->stressyy, E, 1}, {s.strain_rate, f->strain_rate, E, nstr},
+        {s.strain, f->strain, E, nstr}, {s.stress, f->stress, E, nstr},
+    };
+    for (auto &t : tab)
+        if ((rc = upload_soa(ctx, t.dst, t.src, t.n, t.k))) return rc;
+    const double *shp_src[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
+    for (int d = 0; d < nd; ++d)
+        if (shp_src[d])
+            if ((rc = upload_soa(ctx, s.shp + (size_t)d * npe * E, shp_src[d], E, npe))) return rc;
+    double *ev = const_cast<double *>(s.elem_visc), *ec = const_cast<double *>(s.elem_cohesion);
+    if ((rc = field_opt(ctx, &ev, f->elem_visc, E))) return rc;
+    if ((rc = field_opt(ctx, &ec, f->elem_cohesion, E))) return rc;
+    s.elem_visc = ev;
+    s.elem_cohesion = ec;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DES_OK;
+}
+
+int des_download_fields(des_ctx *ctx, des_fields *f)
+{
+    if (!ctx || !f) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    DevState &s = ctx->s;
+    const long N = ctx->N, E = ctx->E;
+    const int nd = ctx->nd, npe = ctx->npe, nstr = ctx->nstr;
+    struct { double *dst; const double *src; long n; int k; } tab[] = {
+        {f->coord, s.coord, N, nd}, {f->coord0, s.coord0, N, nd}, {f->vel, s.vel, N, nd},
+        {f->force, s.force, N, nd}, {f->temperature, s.T, N, 1}, {f->mass, s.mass, N, 1},
+        {f->tmass, s.tmass, N, 1}, {f->volume_n, s.volume_n, N, 1}, {f->ntmp, s.ntmp, N, 1},
+        {f->volume, s.volume, E, 1}, {f->volume_old, s.volume_old, E, 1},
+        {f->edvoldt, s.edvoldt, E, 1}, {f->plstrain, s.plstrain, E, 1},
+        {f->delta_plstrain, s.delta_plstrain, E, 1}, {f->dpressure, s.dpressure, E, 1},
+        {f->stressyy, s.stressyy, E, 1}, {f->strain_rate, s.strain_rate, E, nstr},
+        {f->strain, s.strain, E, nstr}, {f->stress, s.stress, E, nstr},
+    };
+    for (auto &t : tab)
+        if ((rc = download_soa(ctx, t.dst, t.src, t.n, t.k))) return rc;
+    double *shp_dst[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
+    for (int d = 0; d < nd; ++d)
+        if ((rc = download_soa(ctx, shp_dst[d], s.shp + (size_t)d * npe * E, E, npe))) return rc;
+    return DES_OK;
+}
+
+int des_set_scalars(des_ctx *ctx, double dt, double time, int steps, double compensation_pressure)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    DevScalars h{};
+    if ((rc = read_scalars(ctx, h))) return rc;
+    h.dt = dt;
+    h.time = time;
+    h.steps = steps;
+    h.compensation_pressure = compensation_pressure;
+    h.rot_pending = 0;
+    HIPCHK(hipMemcpyAsync(ctx->s.sc, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->steps = steps;
+    return DES_OK;
+}
+
+int des_get_scalars(des_ctx *ctx, double *dt, double *time, int *steps)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    DevScalars h{};
+    int rc = read_scalars(ctx, h);
+    if (rc) return rc;
+    if (dt) *dt = h.dt;
+    if (time) *time = h.time;
+    if (steps) *steps = h.steps;
+    return DES_OK;
+}
+
+#define ENTRY(name, body)                        \
+    int name(des_ctx *ctx)                       \
+    {                                            \
+        if (!ctx) return DES_ERR_INPUT;          \
+        int rc = flush_deferred(ctx);            \
+        if (rc) return rc;                       \
+        body;                                    \
+        return DES_OK;                           \
+    }
+
+ENTRY(des_compute_volume, LAUNCH_ND(k_volume, nblocks(ctx->E), ctx->s))
+ENTRY(des_compute_mass, if ((rc = launch_mass(ctx))) return rc)
+ENTRY(des_compute_shape_fn, LAUNCH_ND(k_shape_fn, nblocks(ctx->E), ctx->s))
+ENTRY(des_update_temperature, LAUNCH_ND(k_temp_elem, nblocks(ctx->E), ctx->s);
+      LAUNCH_ND(k_temp_node, nblocks(ctx->N), ctx->s))
+ENTRY(des_update_strain_rate, LAUNCH_ND(k_strain_rate, nblocks(ctx->E), ctx->s))
+ENTRY(des_compute_dvoldt, LAUNCH_ND(k_dvoldt_node, nblocks(ctx->N), ctx->s))
+ENTRY(des_compute_edvoldt, LAUNCH_ND(k_edvoldt, nblocks(ctx->E), ctx->s))
+ENTRY(des_update_stress, LAUNCH_ND(k_update_stress, nblocks(ctx->E), ctx->s))
+ENTRY(des_nmd_stress, LAUNCH_ND(k_nmd_node, nblocks(ctx->N), ctx->s);
+      LAUNCH_ND(k_nmd_elem, nblocks(ctx->E), ctx->s))
+ENTRY(des_update_force, if ((rc = launch_force(ctx))) return rc)
+ENTRY(des_update_velocity, LAUNCH_ND(k_update_velocity, nblocks(ctx->N), ctx->s))
+ENTRY(des_apply_vbcs, LAUNCH_ND(k_apply_vbcs, nblocks(ctx->N), ctx->s))
+ENTRY(des_update_mesh, if ((rc = launch_update_mesh(ctx))) return rc)
+ENTRY(des_rotate_stress, LAUNCH_ND(k_rotate, nblocks(ctx->E), ctx->s, 0))
+
+int des_init_derived(des_ctx *ctx)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    LAUNCH_ND(k_volume, nblocks(ctx->E), ctx->s);
+    HIPCHK(hipMemcpyAsync(ctx->s.volume_old, ctx->s.volume, sizeof(double) * ctx->E, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = launch_mass(ctx))) return rc;
+    LAUNCH_ND(k_shape_fn, nblocks(ctx->E), ctx->s);
+    return DES_OK;
+}
+
+int des_compute_dt(des_ctx *ctx, double *dt_out)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    if ((rc = launch_compute_dt(ctx))) return rc;
+    DevScalars h{};
+    if ((rc = read_scalars(ctx, h))) return rc;
+    if (dt_out) *dt_out = h.dt;
+    if (h.err) return fail(ctx, h.err, "compute_dt: dt <= 0");
+    return DES_OK;
+}
+
+int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    LAUNCH_ND(k_quality_partial, RED_BLOCKS, ctx->s);
+    std::vector<double> hq(RED_BLOCKS);
+    std::vector<int> hi(RED_BLOCKS);
+    HIPCHK(hipMemcpyAsync(hq.data(), ctx->s.red, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hi.data(), ctx->s.redi, sizeof(int) * RED_BLOCKS, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the reference starts from q = 1, element 0, and keeps the first strict minimum
+    double best = 1;
+    int bi = 0;
+    for (int b = 0; b < RED_BLOCKS; ++b)
+        if (hq[b] < best || (hq[b] == best && best < 1 && hi[b] < bi)) {
+            best = hq[b];
+            bi = hi[b];
+        }
+    if (q) *q = best;
+    if (worst_elem) *worst_elem = bi;
+    return DES_OK;
+}
+
+int des_step(des_ctx *ctx)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->s);
+    if ((rc = check_launch(ctx))) return rc;
+    ctx->steps++;
+    if (ctx->P.has_thermal_diffusion) {
+        LAUNCH_ND(k_temp_elem, nblocks(ctx->E), ctx->s);
+        LAUNCH_ND(k_temp_node, nblocks(ctx->N), ctx->s);
+    }
+    LAUNCH_ND(k_strain_rate, nblocks(ctx->E), ctx->s);
+    LAUNCH_ND(k_dvoldt_node, nblocks(ctx->N), ctx->s);
+    LAUNCH_ND(k_edvoldt, nblocks(ctx->E), ctx->s);
+    LAUNCH_ND(k_update_stress, nblocks(ctx->E), ctx->s);
+    LAUNCH_ND(k_nmd_node, nblocks(ctx->N), ctx->s);
+    LAUNCH_ND(k_nmd_elem, nblocks(ctx->E), ctx->s);
+    if ((rc = launch_force(ctx))) return rc;
+    LAUNCH_ND(k_update_velocity, nblocks(ctx->N), ctx->s);
+    LAUNCH_ND(k_apply_vbcs, nblocks(ctx->N), ctx->s);
+    if ((rc = launch_update_mesh(ctx))) return rc;
+    if (ctx->P.rheol_type & DES_RH_ELASTIC) LAUNCH_ND(k_rotate, nblocks(ctx->E), ctx->s, 0);
+    if (ctx->steps % 10 == 0)
+        if ((rc = launch_compute_dt(ctx))) return rc;
+    return DES_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Timing of the fused passes: an event after each launch on the context's
+// stream; pass k's time is the gap between its event and the previous one.
+struct PassTimer {
+    des_ctx *ctx;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    bool on;
+    explicit PassTimer(des_ctx *c, bool enable) : ctx(c), on(enable) {}
+    ~PassTimer()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    int mark(int k)
+    {
+        if (!on) return DES_OK;
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ev.push_back(e);
+        kind.push_back(k);
+        HIPCHK(hipEventRecord(e, ctx->stream));
+        return DES_OK;
+    }
+    int collect(double *ms, int *counts)
+    {
+        if (!on || ev.empty()) return DES_OK;
+        HIPCHK(hipEventSynchronize(ev.back()));
+        for (size_t i = 1; i < ev.size(); ++i) {
+            if (kind[i] < 0) continue;
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev[i - 1], ev[i]));
+            ms[kind[i]] += t;
+            counts[kind[i]] += 1;
+        }
+        return DES_OK;
+    }
+};
+
+#define MARK(k) do { if ((rc = tm.mark(k))) return rc; } while (0)
+
+int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
+{
+    if (precision != DES_FP64) return fail(ctx, DES_ERR_INPUT, "precision not available");
+    if (nsteps <= 0) return DES_OK;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    const int rotate = (ctx->P.rheol_type & DES_RH_ELASTIC) ? 1 : 0;
+    MARK(-1);
+    for (int k = 0; k < nsteps; ++k) {
+        FusedFlags fl;
+        fl.use_stored_mass = (k == 0);
+        fl.store_aux = (k == nsteps - 1);
+        if (k > 0) swap_volume(ctx);
+        LAUNCH_ND(k_pass_a, nblocks(ctx->E), ctx->s, fl);
+        MARK(0);
+        LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
+        MARK(1);
+        LAUNCH_ND(k_pass_c, nblocks(ctx->E), ctx->s, fl);
+        MARK(2);
+        LAUNCH_ND(k_nmd_node, nblocks(ctx->N), ctx->s);
+        MARK(3);
+        LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
+        MARK(4);
+        if (ctx->nactive) {
+            LAUNCH_ND(k_facet_traction, nblocks(ctx->nactive), ctx->s, ctx->active_facets, ctx->nactive);
+            MARK(5);
+        }
+        LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, fl);
+        MARK(6);
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->s, rotate);
+        if ((rc = check_launch(ctx))) return rc;
+        MARK(-1);
+        ctx->steps++;
+        ctx->closing_pending = true;
+        if (ctx->steps % 10 == 0) {
+            if ((rc = launch_compute_dt(ctx))) return rc;
+            MARK(7);
+        }
+    }
+    return DES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int des_run(des_ctx *ctx, int nsteps, int precision)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    PassTimer tm(ctx, false);
+    return run_fused(ctx, nsteps, precision, tm);
+}
+
+int des_run_timed(des_ctx *ctx, int nsteps, int precision, double *ms, int *counts)
+{
+    if (!ctx || !ms || !counts) return DES_ERR_INPUT;
+    PassTimer tm(ctx, true);
+    int rc = run_fused(ctx, nsteps, precision, tm);
+    if (rc) return rc;
+    return tm.collect(ms, counts);
+}
+
+int des_sync(des_ctx *ctx)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    DevScalars h{};
+    int rc = read_scalars(ctx, h);
+    if (rc) return rc;
+    if (h.err) return fail(ctx, h.err, "device reported an error");
+    return DES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""ctypes binding of libdes_amd.so (the C-ABI in include/des_amd.h).
+
+Loading fails loudly when the library is not built or no HIP device is
+usable: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdes_amd.so")
+
+EXPORTS = (
+    "des_create", "des_destroy", "des_last_error", "des_upload_fields",
+    "des_download_fields", "des_set_scalars", "des_get_scalars",
+    "des_compute_volume", "des_compute_mass", "des_compute_shape_fn",
+    "des_init_derived", "des_update_temperature", "des_update_strain_rate",
+    "des_compute_dvoldt", "des_compute_edvoldt", "des_update_stress",
+    "des_nmd_stress", "des_update_force", "des_update_velocity",
+    "des_apply_vbcs", "des_update_mesh", "des_rotate_stress",
+    "des_compute_dt", "des_worst_quality", "des_step", "des_run", "des_run_timed",
+    "des_sync",
+)
+
+PER_FUNCTION = (
+    "compute_volume", "compute_mass", "compute_shape_fn", "init_derived",
+    "update_temperature", "update_strain_rate", "compute_dvoldt",
+    "compute_edvoldt", "update_stress", "nmd_stress", "update_force",
+    "update_velocity", "apply_vbcs", "update_mesh", "rotate_stress",
+)
+
+_lib = None
+
+
+class DesError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"des status {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """The library handle with argument types set."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run python -m dynearthsol_amd.build")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.des_create.argtypes = [C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
+    lib.des_create.restype = C.c_int
+    lib.des_destroy.argtypes = [vp]
+    lib.des_destroy.restype = None
+    lib.des_last_error.argtypes = [vp]
+    lib.des_last_error.restype = C.c_char_p
+    lib.des_upload_fields.argtypes = [vp, C.POINTER(abi.DesFields)]
+    lib.des_download_fields.argtypes = [vp, C.POINTER(abi.DesFields)]
+    lib.des_set_scalars.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_double]
+    lib.des_get_scalars.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]
+    for f in PER_FUNCTION + ("step", "sync"):
+        getattr(lib, "des_" + f).argtypes = [vp]
+        getattr(lib, "des_" + f).restype = C.c_int
+    lib.des_compute_dt.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.des_worst_quality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.des_run.argtypes = [vp, C.c_int, C.c_int]
+    lib.des_run_timed.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    _lib = lib
+    return lib
+
+
+class DesModel:
+    """One model on the current HIP device, set up like the reference's init()
+    (dynearthsol.cxx:80-98, 316) from a dynearthsol_amd.model.Model."""
+
+    def __init__(self, model, init=True):
+        self.lib = load()
+        self.model = model
+        self.nd, self.npe, self.nstr = model.ndims, model.npe, model.nstr
+        self.N, self.E = model.nnode, model.nelem
+        self.params = model.params
+        self._mesh = model.mesh_struct()
+        ctx = C.c_void_p()
+        rc = self.lib.des_create(C.byref(self.params), C.byref(self._mesh), C.byref(ctx))
+        self.ctx = ctx
+        if rc:
+            msg = self.lib.des_last_error(ctx).decode() if ctx.value else "des_create failed"
+            if ctx.value:
+                self.lib.des_destroy(ctx)
+                self.ctx = C.c_void_p()
+            raise DesError(rc, msg)
+        if init:
+            self.init()
+
+    def close(self):
+        if self.ctx and self.ctx.value:
+            self.lib.des_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise DesError(rc, self.lib.des_last_error(self.ctx).decode())
+
+    def call(self, name):
+        self._check(getattr(self.lib, "des_" + name)(self.ctx))
+
+    def upload(self, **arrays):
+        f = abi.DesFields()
+        keep = []
+        for k, a in arrays.items():
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            setattr(f, k, abi.dptr(a))
+        self._check(self.lib.des_upload_fields(self.ctx, C.byref(f)))
+
+    def init(self):
+        m = self.model
+        self.call("init_derived")          # temperature is still zero: the reference's quirk
+        self.call("apply_vbcs")
+        self.upload(temperature=m.temperature0, stress=m.stress0, strain=m.strain0,
+                    stressyy=m.stressyy0, plstrain=m.plstrain0, elem_visc=m.elem_visc,
+                    elem_cohesion=m.elem_cohesion)
+        self._check(self.lib.des_set_scalars(self.ctx, 0.0, 0.0, 0, m.compensation_pressure))
+        self.compute_dt()
+
+    def compute_dt(self):
+        dt = C.c_double()
+        self._check(self.lib.des_compute_dt(self.ctx, C.byref(dt)))
+        return dt.value
+
+    def step(self, n=1):
+        for _ in range(n):
+            self._check(self.lib.des_step(self.ctx))
+
+    def run(self, n, precision=abi.DES_FP64):
+        self._check(self.lib.des_run(self.ctx, n, precision))
+
+    PASS_NAMES = ("pass_a", "pass_b", "pass_c", "pass_d", "pass_e", "facet", "pass_f", "dt")
+
+    def run_timed(self, n, precision=abi.DES_FP64):
+        """des_run with per-pass HIP-event timing: {pass: (total_ms, launches)}."""
+        ms = (C.c_double * 8)()
+        cnt = (C.c_int * 8)()
+        self._check(self.lib.des_run_timed(self.ctx, n, precision, ms, cnt))
+        return {self.PASS_NAMES[k]: (ms[k], cnt[k]) for k in range(8) if cnt[k]}
+
+    def sync(self):
+        self._check(self.lib.des_sync(self.ctx))
+
+    def scalars(self):
+        dt, t, st = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.lib.des_get_scalars(self.ctx, C.byref(dt), C.byref(t), C.byref(st)))
+        return dt.value, t.value, st.value
+
+    def worst_quality(self):
+        q, w = C.c_double(), C.c_int()
+        self._check(self.lib.des_worst_quality(self.ctx, C.byref(q), C.byref(w)))
+        return q.value, w.value
+
+    def download(self, names=None):
+        """Fields as numpy arrays in the reference's AoS layout."""
+        nd, npe, nstr, N, E = self.nd, self.npe, self.nstr, self.N, self.E
+        shapes = {
+            "coord": (N, nd), "coord0": (N, nd), "vel": (N, nd), "force": (N, nd),
+            "temperature": (N,), "mass": (N,), "tmass": (N,), "volume_n": (N,), "ntmp": (N,),
+            "volume": (E,), "volume_old": (E,), "edvoldt": (E,), "plstrain": (E,),
+            "delta_plstrain": (E,), "dpressure": (E,), "stressyy": (E,),
+            "strain_rate": (E, nstr), "strain": (E, nstr), "stress": (E, nstr),
+            "shpdx": (E, npe), "shpdz": (E, npe),
+        }
+        if nd == 3:
+            shapes["shpdy"] = (E, npe)
+        if names is None:
+            names = list(shapes)
+        out = {n: np.zeros(shapes[n]) for n in names}
+        f = abi.DesFields()
+        for n, a in out.items():
+            setattr(f, n, abi.dptr(a))
+        self._check(self.lib.des_download_fields(self.ctx, C.byref(f)))
+        self.sync()
+        return out

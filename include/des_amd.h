@@ -1,0 +1,210 @@
+/*
+ * des_amd.h -- C-ABI of the MI355X-native DynEarthSol explicit time step.
+ *
+ * The reference (luh1202/DynEarthSol) has no FFI: its per-step physics is a set
+ * of free C++ functions `f(const Param&, const Variables&, <out arrays>)`
+ * called in sequence by the main loop (dynearthsol.cxx:324-406).  This header
+ * replaces that surface with plain C: one opaque context per model, host
+ * arrays in the reference's own AoS layout (array2d.hpp:9-104) at the upload /
+ * download boundary, and one entry point per reference function plus fused
+ * multi-step drivers.  Every entry point returns a status code whose numbering
+ * follows the reference's exit codes (README:139-144) instead of calling
+ * std::exit.
+ *
+ * Dimensions are a run-time choice (p->ndims = 2 or 3) where the reference
+ * fixes them at compile time (constants.hpp:11-23, Makefile:233-238).
+ */
+#ifndef DES_AMD_H
+#define DES_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DES_MAX_MAT 8   /* max material types per model (Mat::nmat) */
+#define DES_NBDRY 10    /* boundary types x0,x1,y0,y1,z0,z1,n0..n3 (constants.hpp:26-36) */
+
+/* Status codes (README:139-144; rheology.cxx:636-639; geometry.cxx:304-308). */
+enum des_status {
+    DES_OK = 0,
+    DES_ERR_INPUT = 1,     /* invalid parameter, unknown rheology or damping option */
+    DES_ERR_IO = 2,
+    DES_ERR_MESH = 10,     /* inconsistent mesh (bad connectivity, node out of range) */
+    DES_ERR_RUNTIME = 11,  /* dt <= 0, non-finite coordinates */
+    DES_ERR_ASSERT = 12,   /* missing edge vector of a boundary intersection */
+    DES_ERR_HIP = 100,     /* HIP runtime failure (message in des_last_error) */
+    DES_ERR_STATE = 101    /* call out of order (e.g. run before upload) */
+};
+
+/* Rheology bit flags (matprops.hpp:47-53). */
+enum des_rheology {
+    DES_RH_ELASTIC = 1,
+    DES_RH_VISCOUS = 2,
+    DES_RH_PLASTIC = 4,
+    DES_RH_MAXWELL = 3,
+    DES_RH_EP = 5,
+    DES_RH_EVP = 7
+};
+
+/* Precision of the fused step (des_run). */
+enum des_precision {
+    DES_FP64 = 0,   /* every field and operation in double */
+    DES_MIXED = 1   /* stress deviator / strain / strain rate / rheology arithmetic in
+                       float; coordinates, volumes, mass, pressure, force accumulation
+                       and the eigen-return in double */
+};
+
+/*
+ * Model parameters: the subset of the reference's Param (parameters.hpp:39-221)
+ * that the explicit step reads, with the reference's names.  Per-material
+ * vectors hold nmat entries (the reference's one-entry broadcast,
+ * matprops.cxx:257-268, is expanded by the caller).
+ */
+typedef struct des_params {
+    int ndims;                    /* 2 or 3 */
+    /* Mat (parameters.hpp:158-186) */
+    int rheol_type;               /* enum des_rheology */
+    int nmat;
+    int is_plane_strain;
+    double visc_min, visc_max, tension_max, therm_diff_max;
+    double rho0[DES_MAX_MAT], alpha[DES_MAX_MAT];
+    double bulk_modulus[DES_MAX_MAT], shear_modulus[DES_MAX_MAT];
+    double visc_exponent[DES_MAX_MAT], visc_coefficient[DES_MAX_MAT];
+    double visc_activation_energy[DES_MAX_MAT];
+    double heat_capacity[DES_MAX_MAT], therm_cond[DES_MAX_MAT];
+    double radiogenic_heating[DES_MAX_MAT];
+    double pls0[DES_MAX_MAT], pls1[DES_MAX_MAT];
+    double cohesion0[DES_MAX_MAT], cohesion1[DES_MAX_MAT];
+    double friction_angle0[DES_MAX_MAT], friction_angle1[DES_MAX_MAT];
+    double dilation_angle0[DES_MAX_MAT], dilation_angle1[DES_MAX_MAT];
+    /* Control (parameters.hpp:79-96) */
+    double gravity, characteristic_speed, inertial_scaling, dt_fraction, fixed_dt;
+    double damping_factor;
+    int damping_option, ref_pressure_option, is_quasi_static, has_thermal_diffusion;
+    int surface_process_option;   /* only 0 (no surface process) is on the device path */
+    /* BC (parameters.hpp:98-135) */
+    double surface_temperature, mantle_temperature;
+    int has_fixed_bottom_temperature;
+    int has_winkler_foundation;
+    double winkler_delta_rho;
+    int has_elastic_foundation;
+    double elastic_foundation_constant;
+    int has_water_loading;
+    int vbc_types[DES_NBDRY];     /* Variables::vbc_types (dynearthsol.cxx:43-52) */
+    double vbc_values[DES_NBDRY]; /* Variables::vbc_values (dynearthsol.cxx:54-63) */
+    /* Mesh (parameters.hpp:54-77): only what the step and its checks read */
+    double zlength;               /* Winkler foundation depth (bc.cxx:532) */
+    double resolution, smallest_size, min_quality;
+    /* Derived by the caller as the reference's init_var does (dynearthsol.cxx:36-39) */
+    double max_vbc_val;
+} des_params;
+
+/*
+ * Mesh and setup artefacts (mesh.cxx:1546-1721, bc.cxx:87-176), host AoS.
+ * Boundary facets are var.bfacets[0..9] concatenated in boundary order.
+ */
+typedef struct des_mesh {
+    int nnode, nelem;
+    const int *connectivity;      /* nelem x (ndims+1) */
+    const double *coord;          /* nnode x ndims: initial coordinates */
+    const unsigned *bcflag;       /* nnode */
+    int nbfacet;
+    const int *bfacet_bdry;       /* nbfacet: boundary index 0..9 */
+    const int *bfacet_elem;       /* nbfacet: element owning the facet */
+    const int *bfacet_face;       /* nbfacet: local facet number (NODE_OF_FACET) */
+    const double *bnormals;       /* DES_NBDRY x ndims */
+    const double *edge_vectors;   /* DES_NBDRY x DES_NBDRY x ndims, entry [i][j], i<j */
+    const int *edge_valid;        /* DES_NBDRY x DES_NBDRY: 1 where the reference map has the pair */
+    const int *elemmarkers;       /* nelem x nmat marker counts, or NULL: every element
+                                     holds markers_per_element markers of material 0 */
+    int markers_per_element;
+} des_mesh;
+
+/*
+ * Field arrays, host AoS in the reference's layout (parameters.hpp:229-279).
+ * Any NULL pointer is skipped by upload and download.
+ */
+typedef struct des_fields {
+    /* node fields */
+    double *coord, *coord0, *vel, *force;           /* nnode x ndims */
+    double *temperature, *mass, *tmass, *volume_n, *ntmp;   /* nnode */
+    /* element fields */
+    double *volume, *volume_old, *edvoldt, *plstrain, *delta_plstrain;
+    double *dpressure, *stressyy;                   /* nelem */
+    double *strain_rate, *strain, *stress;          /* nelem x nstr */
+    double *shpdx, *shpdy, *shpdz;                  /* nelem x (ndims+1) */
+    /* per-element material fields (extension, no counterpart in the reference):
+       viscosity replacing the power law of MatProps::visc, and cohesion replacing
+       cohesion0 in MatProps::plastic_weakening */
+    double *elem_visc, *elem_cohesion;              /* nelem */
+} des_fields;
+
+typedef struct des_ctx des_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Create a model on the current HIP device: uploads the mesh, builds the
+   node->element CSR (create_support, mesh.cxx:1707-1721) and the per-node
+   boundary-facet lists.  Fields start at zero, as allocate_variables leaves
+   them (fields.cxx:10-48). */
+int des_create(const des_params *p, const des_mesh *m, des_ctx **out);
+void des_destroy(des_ctx *ctx);
+const char *des_last_error(const des_ctx *ctx);
+
+int des_upload_fields(des_ctx *ctx, const des_fields *f);
+/* Completes any deferred work of the fused driver first, so downloaded
+   arrays equal the reference's Variables after the same number of steps. */
+int des_download_fields(des_ctx *ctx, des_fields *f);
+
+/* Scalar state: var.dt, var.time, var.steps, var.compensation_pressure. */
+int des_set_scalars(des_ctx *ctx, double dt, double time, int steps,
+                    double compensation_pressure);
+int des_get_scalars(des_ctx *ctx, double *dt, double *time, int *steps);
+
+/* ---- per-function entry points (same semantics as the named function) -- */
+
+int des_compute_volume(des_ctx *ctx);       /* geometry.cxx:86-108 */
+int des_compute_mass(des_ctx *ctx);         /* geometry.cxx:313-365 */
+int des_compute_shape_fn(des_ctx *ctx);     /* geometry.cxx:368-455 */
+/* init()'s derived-state block: compute_volume, volume_old = volume,
+   compute_mass, compute_shape_fn (dynearthsol.cxx:86-91). */
+int des_init_derived(des_ctx *ctx);
+int des_update_temperature(des_ctx *ctx);   /* fields.cxx:96-163 */
+int des_update_strain_rate(des_ctx *ctx);   /* fields.cxx:166-232 */
+int des_compute_dvoldt(des_ctx *ctx);       /* geometry.cxx:111-155 (into ntmp) */
+int des_compute_edvoldt(des_ctx *ctx);      /* geometry.cxx:158-179 (from ntmp) */
+int des_update_stress(des_ctx *ctx);        /* rheology.cxx:510-646 */
+int des_nmd_stress(des_ctx *ctx);           /* geometry.cxx:182-236 */
+int des_update_force(des_ctx *ctx);         /* fields.cxx:291-340 with bc.cxx:493-566 */
+int des_update_velocity(des_ctx *ctx);      /* fields.cxx:343-355 */
+int des_apply_vbcs(des_ctx *ctx);           /* bc.cxx:179-490 */
+int des_update_mesh(des_ctx *ctx);          /* dynearthsol.cxx:221-232 */
+int des_rotate_stress(des_ctx *ctx);        /* fields.cxx:411-483 */
+int des_compute_dt(des_ctx *ctx, double *dt_out);   /* geometry.cxx:239-310; sets var.dt */
+int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem); /* geometry.cxx:499-512 */
+
+/* ---- step drivers --------------------------------------------------------- */
+
+/* One step of the main loop, dynearthsol.cxx:325-361 (no phase changes, markers,
+   output or remeshing), through the per-function kernels above. */
+int des_step(des_ctx *ctx);
+
+/* nsteps steps of the same loop through the fused six-pass schedule, with
+   compute_dt every 10 steps as the reference (dynearthsol.cxx:349-361). */
+int des_run(des_ctx *ctx, int nsteps, int precision);
+
+/* des_run with the duration of each fused pass measured by HIP events on the
+   context's stream: ms[k] accumulates the milliseconds and counts[k] the
+   launches of pass k (0..4 = passes A..E, 5 = facet tractions, 6 = pass F,
+   7 = compute_dt).  Measurement aid with no counterpart in the reference. */
+int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[8], int counts[8]);
+
+/* Wait for all work queued on the context's stream; returns the first device
+   error the step detected (dt <= 0, missing edge vector). */
+int des_sync(des_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DES_AMD_H */
