@@ -993,6 +993,141 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
     }
 }
 
+// ---- mixed precision (DES_MIXED): passes C and E on (pres, sdev) -------
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_to_mixed(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    double st[NSTR], p;
+    float sd[NSTR];
+    load_tensor<ND>(s.stress, E, e, st);
+    split_stress<ND>(st, p, sd);
+    s.pres[e] = p;
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) {
+        s.sdev[k * E + e] = sd[k];
+        s.strain32[k * E + e] = (float)s.strain[k * E + e];
+    }
+    s.pl32[e] = (float)s.plstrain[e];
+    s.dpl32[e] = (float)s.delta_plstrain[e];
+    if (s.visc32) const_cast<float *>(s.visc32)[e] = (float)s.elem_visc[e];
+    if (s.coh32) const_cast<float *>(s.coh32)[e] = (float)s.elem_cohesion[e];
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState s)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    float sd[NSTR];
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) sd[k] = s.sdev[k * E + e];
+    double st[NSTR];
+    join_stress<ND>(s.pres[e], sd, st);
+    store_tensor<ND>(s.stress, E, e, st);
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) s.strain[k * E + e] = (double)s.strain32[k * E + e];
+    s.plstrain[e] = (double)s.pl32[e];
+    s.delta_plstrain[e] = (double)s.dpl32[e];
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevState s, FusedFlags fl)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND], sh[ND][NPE], v[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    const double vol = s.volume[e];
+    shape_fn<ND>(x, vol, sh);
+    gather_vec<ND>(s.vel, s.N, c, v);
+    // velocities relative to node 0 (sum_i dN_i = 0): no cancellation of the
+    // rigid-body part when the rates are rounded to FP32
+#pragma unroll
+    for (int i = NPE - 1; i >= 0; --i)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) v[i][d] -= v[0][d];
+    double p = s.pres[e];
+    float sd[NSTR], es[NSTR], edot[NSTR];
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) {
+        sd[k] = s.sdev[k * E + e];
+        es[k] = s.strain32[k * E + e];
+    }
+    if (s.sc->rot_pending) {
+        double w[3];
+        spin<ND>(v, sh, w);
+        const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
+        const float dtp = (float)s.sc->dt_prev;
+        jaumann_t<ND, float>(sd, dtp, wf);
+        jaumann_t<ND, float>(es, dtp, wf);
+    }
+    {
+        double sr[NSTR];
+        strain_rate<ND>(v, sh, sr);
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) edot[k] = (float)sr[k];
+    }
+    double edv = 0;
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) edv += s.ntmp[c[i]];
+    edv = edv / NPE;
+    float pl = s.pl32[e], dpl = s.dpl32[e];
+    double dp;
+    const float Tm = (float)elem_mean<ND>(s.T, c);
+    bool ok = stress_update_mixed<ND>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p, sd,
+                                      es, edot, pl, dpl, dp);
+    if (!ok) set_error(s.sc, DES_ERR_INPUT);
+    s.pres[e] = p;
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) {
+        s.sdev[k * E + e] = sd[k];
+        s.strain32[k * E + e] = es[k];
+    }
+    s.pl32[e] = pl;
+    s.dpl32[e] = dpl;
+    s.dpressure[e] = dp;
+    if (fl.store_aux) {
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) s.strain_rate[k * E + e] = (double)edot[k];
+        s.edvoldt[e] = edv;
+    }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
+{
+    int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double dpn = 0;
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) dpn += s.ntmp[c[i]];
+    const double dp_el = dpn / NPE;
+    double p = s.pres[e] + (-s.dpressure[e] + dp_el) / ND;   // NMD_stress on the diagonal mean
+    s.pres[e] = p;
+    float sd[NSTR];
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) sd[k] = s.sdev[k * E + e];
+    double st[NSTR];
+    join_stress<ND>(p, sd, st);
+    double x[NPE][ND], sh[ND][NPE], out[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    const double vol = s.volume[e];
+    shape_fn<ND>(x, vol, sh);
+    force_contrib<ND>(s, e, vol, sh, st, elem_mean<ND>(s.T, c), out);
+    store_force_contrib<ND>(s, e, out);
+}
+
 // step bookkeeping after pass F: var.time, var.steps, the rotation to defer
 __global__ void k_step_end(DevState s, int rotate)
 {
@@ -1629,12 +1764,45 @@ struct PassTimer {
 
 #define MARK(k) do { if ((rc = tm.mark(k))) return rc; } while (0)
 
+// device buffers of the mixed-precision state, allocated on first use
+int ensure_mixed(des_ctx *ctx)
+{
+    DevState &s = ctx->s;
+    if (s.pres) return DES_OK;
+    const size_t E = ctx->E, nstr = ctx->nstr;
+    int rc;
+    if ((rc = dalloc_n(ctx, &s.pres, E))) return rc;
+    if ((rc = dalloc_n(ctx, &s.sdev, E * nstr))) return rc;
+    if ((rc = dalloc_n(ctx, &s.strain32, E * nstr))) return rc;
+    if ((rc = dalloc_n(ctx, &s.pl32, E))) return rc;
+    if ((rc = dalloc_n(ctx, &s.dpl32, E))) return rc;
+    if (s.elem_visc) {
+        float *v;
+        if ((rc = dalloc_n(ctx, &v, E))) return rc;
+        s.visc32 = v;
+    }
+    if (s.elem_cohesion) {
+        float *v;
+        if ((rc = dalloc_n(ctx, &v, E))) return rc;
+        s.coh32 = v;
+    }
+    return DES_OK;
+}
+
 int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
 {
-    if (precision != DES_FP64) return fail(ctx, DES_ERR_INPUT, "precision not available");
+    if (precision != DES_FP64 && precision != DES_MIXED)
+        return fail(ctx, DES_ERR_INPUT, "unknown precision");
+    if (precision == DES_MIXED && ctx->P.is_plane_strain)
+        return fail(ctx, DES_ERR_INPUT, "mixed precision does not cover plane strain");
     if (nsteps <= 0) return DES_OK;
     int rc = flush_deferred(ctx);
     if (rc) return rc;
+    const bool mixed = precision == DES_MIXED;
+    if (mixed) {
+        if ((rc = ensure_mixed(ctx))) return rc;
+        LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);
+    }
     const int rotate = (ctx->P.rheol_type & DES_RH_ELASTIC) ? 1 : 0;
     MARK(-1);
     for (int k = 0; k < nsteps; ++k) {
@@ -1646,11 +1814,13 @@ int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
         MARK(0);
         LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
         MARK(1);
-        LAUNCH_ND(k_pass_c, nblocks(ctx->E), ctx->s, fl);
+        if (mixed) LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
+        else LAUNCH_ND(k_pass_c, nblocks(ctx->E), ctx->s, fl);
         MARK(2);
         LAUNCH_ND(k_nmd_node, nblocks(ctx->N), ctx->s);
         MARK(3);
-        LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
+        if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
+        else LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
         MARK(4);
         if (ctx->nactive) {
             LAUNCH_ND(k_facet_traction, nblocks(ctx->nactive), ctx->s, ctx->active_facets, ctx->nactive);
@@ -1668,6 +1838,7 @@ int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
             MARK(7);
         }
     }
+    if (mixed) LAUNCH_ND(k_from_mixed, nblocks(ctx->E), ctx->s);
     return DES_OK;
 }
 

@@ -66,6 +66,12 @@ struct DevState {
     DevScalars *sc;
     double *red;                                  // reduction partials
     int *redi;
+    // mixed-precision state (des_run with DES_MIXED): stress = sdev + pres * I
+    double *pres;                                 // E, mean of the stress diagonal
+    float *sdev;                                  // NSTR x E, stress minus pres * I
+    float *strain32;                              // NSTR x E
+    float *pl32, *dpl32;                          // E
+    const float *visc32, *coh32;                  // E or null
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
@@ -672,13 +678,12 @@ template <int ND> __device__ __forceinline__ void viscous(double bulkm, double v
     for (int i = ND; i < Dim<ND>::NSTR; ++i) s[i] = 2 * viscosity * edot[i];
 }
 
-// elasto_plastic: Mohr-Coulomb return in principal space (rheology.cxx:151-299)
-template <int ND> __device__ __forceinline__ void elasto_plastic(double bulkm, double shearm, double amc,
-                                                                double anphi, double anpsi, double hardn,
-                                                                double ten_max, const double *de,
-                                                                double &depls, double *s)
+// Mohr-Coulomb return in principal space of an elastic trial stress s
+// (rheology.cxx:167-298)
+template <int ND> __device__ __forceinline__ void mc_return(double bulkm, double shearm, double amc,
+                                                           double anphi, double anpsi, double hardn,
+                                                           double ten_max, double &depls, double *s)
 {
-    elastic<ND>(bulkm, shearm, de, s);
     depls = 0;
     double p[ND];
     double v[3][3];
@@ -730,6 +735,16 @@ template <int ND> __device__ __forceinline__ void elasto_plastic(double bulkm, d
         s[1] = 0.5 * (dss - dc2);
         s[2] = 0.5 * (p[0] - p[1]) * sin2t;
     }
+}
+
+// elasto_plastic: elastic trial then Mohr-Coulomb return (rheology.cxx:151-299)
+template <int ND> __device__ __forceinline__ void elasto_plastic(double bulkm, double shearm, double amc,
+                                                                double anphi, double anpsi, double hardn,
+                                                                double ten_max, const double *de,
+                                                                double &depls, double *s)
+{
+    elastic<ND>(bulkm, shearm, de, s);
+    mc_return<ND>(bulkm, shearm, amc, anphi, anpsi, hardn, ten_max, depls, s);
 }
 
 // geoFLAC-style plane-strain Mohr-Coulomb (rheology.cxx:302-507)
@@ -915,6 +930,252 @@ template <int ND> __device__ __forceinline__ bool stress_update(const DevState &
         ok = false;
     }
     dpressure = trace<ND>(s) - old_s;
+    return ok;
+}
+
+// ===================================================== mixed precision
+//
+// The stress is carried as an FP64 mean p of its diagonal plus an FP32
+// remainder sd (the deviator); strain, plastic strain and strain rate are
+// FP32.  The rheology arithmetic on sd, the strain and the material
+// properties runs in FP32; the pressure, the volume change and the
+// Mohr-Coulomb eigen-return run in FP64.  At ~3e8 Pa of lithostatic pressure
+// an FP32 total stress would round away the ~1e2-1e3 Pa per-step increments.
+
+template <int ND> __device__ __forceinline__ float trace_f(const float *s)
+{
+    if constexpr (ND == 3) return s[0] + s[1] + s[2];
+    else return s[0] + s[1];
+}
+
+template <int ND> __device__ __forceinline__ float second_invariant2_f(const float *t)
+{
+    if constexpr (ND == 3) {
+        float a = (t[0] + t[1] + t[2]) / 3;
+        return (0.5f * ((t[0] - a) * (t[0] - a) + (t[1] - a) * (t[1] - a) + (t[2] - a) * (t[2] - a))
+                + t[3] * t[3] + t[4] * t[4] + t[5] * t[5]);
+    } else {
+        return 0.25f * (t[0] - t[1]) * (t[0] - t[1]) + t[2] * t[2];
+    }
+}
+
+template <int ND, typename T> __device__ __forceinline__ void jaumann_t(T *s, T dt, const T w[3])
+{
+    if constexpr (ND == 3) {
+        const T w3 = w[0], w4 = w[1], w5 = w[2];
+        T s_inc[6];
+        s_inc[0] = T(-2) * s[3] * w3 - T(2) * s[4] * w4;
+        s_inc[1] = T(2) * s[3] * w3 - T(2) * s[5] * w5;
+        s_inc[2] = T(2) * s[4] * w4 + T(2) * s[5] * w5;
+        s_inc[3] = s[0] * w3 - s[1] * w3 - s[4] * w5 - s[5] * w4;
+        s_inc[4] = s[0] * w4 - s[2] * w4 + s[3] * w5 - s[5] * w3;
+        s_inc[5] = s[1] * w5 - s[2] * w5 + s[3] * w4 + s[4] * w3;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] += dt * s_inc[i];
+    } else {
+        const T w2 = w[0];
+        T s_inc[3];
+        s_inc[0] = T(-2) * s[2] * w2;
+        s_inc[1] = T(2) * s[2] * w2;
+        s_inc[2] = s[0] * w2 - s[1] * w2;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s[i] += dt * s_inc[i];
+    }
+}
+
+// MatProps::visc in FP32 (power law of matprops.cxx:275-311, same clamp)
+template <int ND> __device__ __forceinline__ float mat_visc_f(const DevState &s, int e, float Tmean,
+                                                            const float *edot)
+{
+    const des_params *P = s.P;
+    float visc;
+    if (s.visc32) {
+        visc = s.visc32[e];
+    } else {
+        float ed = sqrtf(second_invariant2_f<ND>(edot));
+        ed = fmaxf(ed, 1e-30f);
+        float result = 0;
+        int n = 0;
+        for (int m = 0; m < P->nmat; m++) {
+            const float ne = (float)P->visc_exponent[m];
+            float visc0 = 0.25f * powf(ed, 1 / ne - 1) * powf(0.75f * (float)P->visc_coefficient[m], -1 / ne)
+                * expf((float)P->visc_activation_energy[m] / (ne * 8.3144f * Tmean)) * 1e6f;
+            int k = count_of(s, e, m);
+            result += k / visc0;
+            n += k;
+        }
+        visc = n / result;
+    }
+    const float vmin = (float)P->visc_min, vmax = (float)P->visc_max;
+    return fminf(fmaxf(visc, vmin), vmax);
+}
+
+// plastic_weakening + plastic_props in FP32 (matprops.cxx:314-372)
+__device__ __forceinline__ void mat_plastic_props_f(const DevState &s, int e, float pls, double &amc,
+                                                    double &anphi, double &anpsi, double &hardn,
+                                                    double &ten_max)
+{
+    const des_params *P = s.P;
+    float c = 0, f = 0, d = 0, h = 0;
+    int n = 0;
+    for (int m = 0; m < P->nmat; m++) {
+        int k = count_of(s, e, m);
+        if (k == 0) continue;
+        n += k;
+        const float c0 = s.coh32 ? s.coh32[e] : (float)P->cohesion0[m];
+        const float c1 = (float)P->cohesion1[m];
+        const float p0 = (float)P->pls0[m], p1 = (float)P->pls1[m];
+        const float f0 = (float)P->friction_angle0[m], f1 = (float)P->friction_angle1[m];
+        const float d0 = (float)P->dilation_angle0[m], d1 = (float)P->dilation_angle1[m];
+        if (pls < p0) {
+            c += c0 * k; f += f0 * k; d += d0 * k;
+        } else if (pls < p1) {
+            float p = (pls - p0) / (p1 - p0);
+            c += (c0 + p * (c1 - c0)) * k;
+            f += (f0 + p * (f1 - f0)) * k;
+            d += (d0 + p * (d1 - d0)) * k;
+            h += (c1 - c0) / (p1 - p0) * k;
+        } else {
+            c += c1 * k; f += f1 * k; d += d1 * k;
+        }
+    }
+    float cohesion = c / n, phi = f / n, psi = d / n;
+    hardn = h / n;
+    const float deg = (float)DEG2RAD;
+    float sphi = sinf(phi * deg), spsi = sinf(psi * deg);
+    float aphi = (1 + sphi) / (1 - sphi), apsi = (1 + spsi) / (1 - spsi);
+    anphi = aphi;
+    anpsi = apsi;
+    amc = 2 * cohesion * sqrtf(aphi);
+    ten_max = (phi == 0) ? P->tension_max : dmin(P->tension_max, (double)(cohesion / tanf(phi * deg)));
+}
+
+// elastic increment of (p, sd): the diagonal mean takes (lambda + 2G/ND) tr(de)
+template <int ND> __device__ __forceinline__ void elastic_split(float K, float G, const float *de,
+                                                               double &p, float *sd)
+{
+    const float lambda = K - 2.f / 3 * G;
+    const float tr = trace_f<ND>(de), mean = tr / ND;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) sd[i] += 2 * G * (de[i] - mean);
+#pragma unroll
+    for (int i = ND; i < Dim<ND>::NSTR; ++i) sd[i] += 2 * G * de[i];
+    p += (double)((lambda + 2 * G / ND) * tr);
+}
+
+// maxwell (rheology.cxx:118-134) on (p, sd): the mean of the diagonal only
+// takes the bulk term
+template <int ND> __device__ __forceinline__ void maxwell_split(float K, float G, float eta, float dt,
+                                                               double dv, const float *de, double &p,
+                                                               float *sd)
+{
+    float tmp = 0.5f * dt * G / eta;
+    float f1 = 1 - tmp, f2 = 1 / (1 + tmp);
+    float dev = trace_f<ND>(de) / ND;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) sd[i] = (sd[i] * f1 + 2 * G * (de[i] - dev)) * f2;
+#pragma unroll
+    for (int i = ND; i < Dim<ND>::NSTR; ++i) sd[i] = (sd[i] * f1 + 2 * G * de[i]) * f2;
+    p += (double)K * dv;
+}
+
+template <int ND> __device__ __forceinline__ void join_stress(double p, const float *sd, double *s)
+{
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s[i] = (double)sd[i] + p;
+#pragma unroll
+    for (int i = ND; i < Dim<ND>::NSTR; ++i) s[i] = (double)sd[i];
+}
+
+template <int ND> __device__ __forceinline__ void split_stress(const double *s, double &p, float *sd)
+{
+    p = trace<ND>(s) / ND;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) sd[i] = (float)(s[i] - p);
+#pragma unroll
+    for (int i = ND; i < Dim<ND>::NSTR; ++i) sd[i] = (float)s[i];
+}
+
+// update_stress for one element in mixed precision (plane strain excluded)
+template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf,
+                                                                     float edvoldt, float Tmean, double vol,
+                                                                     double vol_old, double &p, float *sd,
+                                                                     float *es, float *edot, float &pl,
+                                                                     float &dpl, double &dpressure)
+{
+    constexpr int NSTR = Dim<ND>::NSTR;
+    const des_params *P = st.P;
+    const double p_old = p;
+    {
+        float div = trace_f<ND>(edot);
+#pragma unroll
+        for (int i = 0; i < ND; ++i) edot[i] += (edvoldt - div) / ND;
+    }
+#pragma unroll
+    for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dtf;
+    float de[NSTR];
+#pragma unroll
+    for (int i = 0; i < NSTR; ++i) de[i] = edot[i] * dtf;
+    const float K = (float)harmonic_mean(st, P->bulk_modulus, e);
+    const float G = (float)harmonic_mean(st, P->shear_modulus, e);
+    bool ok = true;
+    switch (P->rheol_type) {
+    case DES_RH_ELASTIC:
+        elastic_split<ND>(K, G, de, p, sd);
+        break;
+    case DES_RH_VISCOUS: {
+        float eta = mat_visc_f<ND>(st, e, Tmean, edot);
+        float dev = trace_f<ND>(edot) / ND;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) sd[i] = 2 * eta * (edot[i] - dev);
+#pragma unroll
+        for (int i = ND; i < NSTR; ++i) sd[i] = 2 * eta * edot[i];
+        p = (double)K * (double)trace_f<ND>(es);
+        break;
+    }
+    case DES_RH_MAXWELL: {
+        float eta = mat_visc_f<ND>(st, e, Tmean, edot);
+        maxwell_split<ND>(K, G, eta, dtf, vol / vol_old - 1, de, p, sd);
+        break;
+    }
+    case DES_RH_EP:
+    case DES_RH_EVP: {
+        double pv = p;
+        float sv[NSTR];
+        float svII = 0;
+        if (P->rheol_type == DES_RH_EVP) {
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) sv[i] = sd[i];
+            float eta = mat_visc_f<ND>(st, e, Tmean, edot);
+            maxwell_split<ND>(K, G, eta, dtf, vol / vol_old - 1, de, pv, sv);
+            svII = second_invariant2_f<ND>(sv);
+        }
+        double amc, anphi, anpsi, hardn, ten_max;
+        mat_plastic_props_f(st, e, pl, amc, anphi, anpsi, hardn, ten_max);
+        double pp = p;
+        float sp[NSTR];
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) sp[i] = sd[i];
+        elastic_split<ND>(K, G, de, pp, sp);
+        double s64[NSTR];
+        join_stress<ND>(pp, sp, s64);
+        double depls;
+        mc_return<ND>((double)K, (double)G, amc, anphi, anpsi, hardn, ten_max, depls, s64);
+        if (P->rheol_type == DES_RH_EVP && (double)svII < second_invariant2<ND>(s64)) {
+            p = pv;
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) sd[i] = sv[i];
+        } else {
+            split_stress<ND>(s64, p, sd);
+            pl += (float)depls;
+            dpl = (float)depls;
+        }
+        break;
+    }
+    default:
+        ok = false;
+    }
+    dpressure = ND * (p - p_old);
     return ok;
 }
 

@@ -1,0 +1,405 @@
+"""The oracle pinned by mathematics: closed forms, invariants and textbook
+routines, chosen so that a dropped term, wrong sign, index or transposed
+operand fails at least one of them."""
+import ctypes as C
+import math
+import types
+
+import numpy as np
+import pytest
+
+import scenarios as SC
+from dynearthsol_amd import abi, model as M
+from oracle import OracleState, oracle_lib
+
+_d = C.POINTER(C.c_double)
+
+
+def _eig(a6):
+    lib = oracle_lib()
+    a6 = np.ascontiguousarray(a6, dtype=np.float64)
+    w, q = np.zeros(3), np.zeros(9)
+    r = lib.o3_dsyevh3(a6.ctypes.data_as(_d), w.ctypes.data_as(_d), q.ctypes.data_as(_d))
+    return r, w, q.reshape(3, 3)
+
+
+def _principal(s6):
+    lib = oracle_lib()
+    s6 = np.ascontiguousarray(s6, dtype=np.float64)
+    p, v = np.zeros(3), np.zeros(9)
+    lib.o3_principal_stresses3(s6.ctypes.data_as(_d), p.ctypes.data_as(_d), v.ctypes.data_as(_d))
+    return p, v.reshape(3, 3)
+
+
+def _full(a6):
+    return np.array([[a6[0], a6[3], a6[4]], [a6[3], a6[1], a6[5]], [a6[4], a6[5], a6[2]]])
+
+
+# ------------------------------------------------------------ eigen solver
+
+@pytest.mark.parametrize("a6", [
+    [3e4, -1, 3, 2, 4, 2],           # tests.cxx:83-90
+    [3e4, -1e-5, 3, 2, 4, 8],        # tests.cxx:136-142
+    [1.0, 2.0, 3.0, 0.1, 0.2, 0.3],
+    [-5e7, -6e7, -8e7, 1e6, -2e6, 3e5],
+])
+def test_eigen_against_lapack(a6):
+    """Eigenvalues against LAPACK (numpy.linalg.eigh) and V diag(w) V^T = A."""
+    r, w, Q = _eig(a6)
+    A = _full(a6)
+    ref = np.linalg.eigvalsh(A)
+    scale = np.max(np.abs(ref))
+    assert r == 0
+    assert np.max(np.abs(np.sort(w) - ref)) <= 1e-11 * scale
+    assert np.max(np.abs(Q @ np.diag(w) @ Q.T - A)) <= 1e-10 * scale
+    assert np.max(np.abs(Q.T @ Q - np.eye(3))) <= 1e-10
+
+
+def test_eigen_degenerate_takes_ql_fallback():
+    """An isotropic tensor makes the cross-product vectors vanish: the QL
+    fallback (dsyevq3) must return the exact eigensystem."""
+    r, w, Q = _eig([-3e8, -3e8, -3e8, 0, 0, 0])
+    assert r == 0
+    assert np.all(w == -3e8)
+    assert np.max(np.abs(Q.T @ Q - np.eye(3))) <= 1e-15
+    r, w, Q = _eig([2.0, 2.0, 5.0, 0.0, 0.0, 0.0])
+    assert np.allclose(np.sort(w), [2, 2, 5], rtol=0, atol=1e-14)
+
+
+def test_principal_stresses_sorted_and_paired():
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        s6 = rng.normal(0, 1e7, 6)
+        p, V = _principal(s6)
+        assert p[0] <= p[1] <= p[2]
+        A = _full(s6)
+        for k in range(3):   # column k is the eigenvector of p[k]
+            assert np.linalg.norm(A @ V[:, k] - p[k] * V[:, k]) <= 1e-7 * np.max(np.abs(p))
+
+
+# ------------------------------------------------------ tiny hand-built states
+
+def _tiny_model(coord, conn, cfg_overrides=None, ndims=3):
+    """A Model-like object for one or a few hand-built elements."""
+    cfg = M.resolve_cfg({**SC.BASE, "mesh.xlength": 1.0, "mesh.ylength": 1.0, "mesh.zlength": 1.0,
+                         "mesh.resolution": 1.0, **(cfg_overrides or {})})
+    m = types.SimpleNamespace()
+    m.params = M.make_params(cfg, ndims)
+    m.ndims, m.npe, m.nstr = ndims, ndims + 1, 6 if ndims == 3 else 3
+    m.coord = np.ascontiguousarray(coord, dtype=np.float64)
+    m.conn = np.ascontiguousarray(conn, dtype=np.int32)
+    m.nnode, m.nelem = len(coord), len(conn)
+    m.bcflag = np.zeros(m.nnode, dtype=np.uint32)
+    m.bfacets = (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    m.bnormals = np.zeros((10, ndims))
+    m.edge_vectors = np.zeros((10, 10, ndims))
+    m.edge_valid = np.zeros((10, 10), dtype=np.int32)
+    m.markers_per_element = 4
+    m.elemmarkers = None
+    m.elem_visc = m.elem_cohesion = None
+    return m
+
+
+def test_regular_tetrahedron_volume():
+    s = 2.5
+    reg = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float) * (s / (2 * math.sqrt(2)))
+    for conn in ([0, 1, 2, 3], [0, 2, 1, 3]):
+        o = OracleState(_tiny_model(reg, [conn]), init=False)
+        o.call("compute_volume")
+        v = o.f["volume"][0]
+        assert abs(abs(v) - s ** 3 / (6 * math.sqrt(2))) <= 1e-14 * s ** 3
+    # the two orderings have opposite signed volumes (geometry.cxx:29-49 is signed)
+    o1 = OracleState(_tiny_model(reg, [[0, 1, 2, 3]]), init=False)
+    o2 = OracleState(_tiny_model(reg, [[0, 2, 1, 3]]), init=False)
+    o1.call("compute_volume")
+    o2.call("compute_volume")
+    assert o1.f["volume"][0] == -o2.f["volume"][0]
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2"])
+def test_shape_functions_linear_completeness(name):
+    """sum_i dN_i/dx_a = 0 and sum_i x_i^b dN_i/dx_a = delta_ab on every element."""
+    m = SC.build(name, nelem=800)
+    o = OracleState(m)
+    nd = m.ndims
+    shp = [o.f["shpdx"]] + ([o.f["shpdy"]] if nd == 3 else []) + [o.f["shpdz"]]
+    X = o.f["coord"][m.conn]                     # E x NPE x ND
+    for a in range(nd):
+        h = np.max(np.abs(shp[a]), axis=1)
+        assert np.max(np.abs(shp[a].sum(axis=1)) / h) <= 1e-13
+        for b in range(nd):
+            s = np.sum(X[:, :, b] * shp[a], axis=1)
+            assert np.max(np.abs(s - (1.0 if a == b else 0.0))) <= 1e-11
+    assert abs(o.f["volume"].sum() - np.prod(m.lengths)) <= 1e-12 * np.prod(m.lengths)
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2"])
+def test_strain_rate_and_spin_of_linear_velocity(name):
+    """v = L x gives edot = sym(L) exactly, and the Jaumann update equals
+    dt (S W - W S) with W = skew part (fields.cxx:413-427 conventions)."""
+    m = SC.build(name, nelem=600)
+    o = OracleState(m)
+    nd = m.ndims
+    rng = np.random.default_rng(1)
+    L = rng.normal(0, 1e-14, (nd, nd))
+    o.f["vel"][:] = o.f["coord"] @ L.T
+    o.call("update_strain_rate")
+    sr = o.f["strain_rate"]
+    if nd == 3:
+        want = [L[0, 0], L[1, 1], L[2, 2], (L[0, 1] + L[1, 0]) / 2, (L[0, 2] + L[2, 0]) / 2,
+                (L[1, 2] + L[2, 1]) / 2]
+    else:
+        want = [L[0, 0], L[1, 1], (L[0, 1] + L[1, 0]) / 2]
+    for k, wv in enumerate(want):
+        assert np.max(np.abs(sr[:, k] - wv)) <= 1e-9 * np.max(np.abs(L))
+    # rotation of a known stress
+    s0 = rng.normal(0, 1e7, (m.nelem, m.nstr))
+    o.f["stress"][:] = s0
+    o.f["strain"][:] = 0
+    o.S.dt = dt = 1e10
+    o.call("rotate_stress")
+    ds = (o.f["stress"] - s0) / dt
+    if nd == 3:
+        w3 = 0.5 * (L[0, 1] - L[1, 0])
+        w4 = 0.5 * (L[0, 2] - L[2, 0])
+        w5 = 0.5 * (L[1, 2] - L[2, 1])
+        W = np.array([[0, w3, w4], [-w3, 0, w5], [-w4, -w5, 0]])
+        idx = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    else:
+        w2 = 0.5 * (L[1, 0] - L[0, 1])     # fields.cxx:475 (z-x convention of the 2D code)
+        W = np.array([[0, w2], [-w2, 0]])
+        idx = [(0, 0), (1, 1), (0, 1)]
+    for e in range(0, m.nelem, 37):
+        S = np.zeros((nd, nd))
+        for k, (i, j) in enumerate(idx):
+            S[i, j] = S[j, i] = s0[e, k]
+        R = S @ W - W @ S
+        got = ds[e]
+        want = np.array([R[i, j] for i, j in idx])
+        assert np.max(np.abs(got - want)) <= 1e-6 * np.max(np.abs(want)) + 1e-30
+        assert abs(sum(got[:nd])) <= 1e-6 * np.max(np.abs(want)) + 1e-30   # trace(SW - WS) = 0
+
+
+def _stress_only_state(name, rheology, extra=None):
+    cfg_extra = {"mat.rheology_type": rheology, **(extra or {})}
+    m = SC.build(name, nelem=600)
+    m.params = M.make_params(M.resolve_cfg({**m.cfg, **cfg_extra}), m.ndims)
+    o = OracleState(m)
+    return m, o
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2"])
+def test_elastic_is_hookes_law(name):
+    m, o = _stress_only_state(name, "elastic")
+    nd, nstr = m.ndims, m.nstr
+    rng = np.random.default_rng(2)
+    o.f["strain_rate"][:] = rng.normal(0, 1e-14, (m.nelem, nstr))
+    o.f["edvoldt"][:] = o.f["strain_rate"][:, :nd].sum(axis=1)   # anti-locking is a no-op
+    s0 = o.f["stress"].copy()
+    de = o.f["strain_rate"] * o.S.dt
+    o.call("update_stress")
+    K, G = m.params.bulk_modulus[0], m.params.shear_modulus[0]
+    lam = K - 2.0 / 3 * G
+    want = s0 + 2 * G * de
+    want[:, :nd] += lam * de[:, :nd].sum(axis=1)[:, None]
+    assert np.max(np.abs(o.f["stress"] - want)) <= 1e-9 * np.max(np.abs(want - s0))
+    assert np.allclose(o.f["dpressure"], (o.f["stress"][:, :nd] - s0[:, :nd]).sum(axis=1), rtol=1e-12, atol=1e-3)
+
+
+def test_maxwell_stress_buildup_matches_closed_form():
+    """Constant deviatoric strain rate from zero stress: tau(t) = 2 edot eta
+    (1 - exp(-G t / eta)) (benchmarks/maxwell.diff:59-65)."""
+    eta = 1e22
+    m, o = _stress_only_state("evp3", "maxwell", {"mat.min_viscosity": eta, "mat.max_viscosity": eta})
+    G = m.params.shear_modulus[0]
+    edot = 1e-14
+    o.f["stress"][:] = 0
+    o.S.dt = dt = 0.02 * eta / G
+    nsteps = 200
+    for _ in range(nsteps):
+        o.f["strain_rate"][:] = 0
+        o.f["strain_rate"][:, 3] = edot         # pure shear xy
+        o.f["edvoldt"][:] = 0
+        o.f["volume_old"][:] = o.f["volume"]
+        o.call("update_stress")
+    t = nsteps * dt
+    want = 2 * edot * eta * (1 - math.exp(-G * t / eta))
+    got = o.f["stress"][:, 3]
+    assert np.max(np.abs(got - want)) <= 1e-3 * want
+    assert np.max(np.abs(o.f["stress"][:, :3])) <= 1e-6 * want
+
+
+def test_mohr_coulomb_return_lands_on_yield_surface():
+    """With saturated softening (no hardening), a shear return leaves the
+    principal stresses on fs = s1 - s3 N_phi + 2 c sqrt(N_phi) = 0, with the
+    plastic increments in the ratio set by the dilation angle."""
+    extra = {"mat.pls0": [0.0], "mat.pls1": [1e-9], "mat.dilation_angle1": [10.0]}
+    m, o = _stress_only_state("evp3", "elasto-plastic", extra)
+    o.f["plstrain"][:] = 1.0
+    o.f["strain_rate"][:] = 0
+    o.f["edvoldt"][:] = 0
+    rng = np.random.default_rng(3)
+    p = -rng.uniform(1e6, 5e7, m.nelem)
+    s = np.zeros((m.nelem, 6))
+    s[:, :3] = p[:, None]
+    s += rng.normal(0, 3e7, (m.nelem, 6))
+    o.f["stress"][:] = s
+    o.call("update_stress")
+    c = m.params.cohesion1[0]
+    phi = m.params.friction_angle1[0] * math.pi / 180
+    psi = 10.0 * math.pi / 180
+    nphi = (1 + math.sin(phi)) / (1 - math.sin(phi))
+    npsi = (1 + math.sin(psi)) / (1 - math.sin(psi))
+    K, G = m.params.bulk_modulus[0], m.params.shear_modulus[0]
+    a1, a2 = K + 4.0 / 3 * G, K - 2.0 / 3 * G
+    ten = min(m.params.tension_max, c / math.tan(phi))
+    checked = 0
+    for e in range(m.nelem):
+        if o.f["delta_plstrain"][e] == 0:
+            continue
+        # principal values of the returned stress in the trial eigenframe (the
+        # return may reorder them, so they are not re-sorted)
+        trial, V = np.linalg.eigh(_full(s[e]))
+        new = np.einsum("ik,ij,jk->k", V, _full(o.f["stress"][e]), V)
+        pa = math.sqrt(1 + nphi * nphi) + nphi
+        h = trial[2] - ten + pa * (trial[0] - (ten * nphi - 2 * c * math.sqrt(nphi)))
+        if h >= 0:
+            continue     # tensile return
+        fs = new[0] - new[2] * nphi + 2 * c * math.sqrt(nphi)
+        assert abs(fs) <= 1e-6 * np.max(np.abs(trial))
+        d0, d2 = new[0] - trial[0], new[2] - trial[2]
+        assert abs(d0 * (a2 - a1 * npsi) - d2 * (a1 - a2 * npsi)) <= 1e-6 * abs(d0 * (a2 - a1 * npsi))
+        checked += 1
+    assert checked > 50
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2"])
+def test_uniform_stress_patch_test_and_gravity_balance(name):
+    """Uniform stress gives zero force at interior nodes; the sum of all nodal
+    forces is minus the weight of the model (no traction BCs, no damping)."""
+    m = SC.build(name, nelem=1500)
+    m.params.damping_option = 0
+    o = OracleState(m)
+    o.f["stress"][:] = np.array([-3e7, -2e7, -4e7, 5e6, -1e6, 2e6][:m.nstr]) if m.ndims == 3 \
+        else np.array([-3e7, -4e7, 5e6])
+    grav = m.params.gravity
+    o.call("update_force")
+    f = o.f["force"]
+    interior = m.bcflag == 0
+    typical = 3e7 * np.prod(m.lengths) / m.nelem ** ((m.ndims - 1) / m.ndims)
+    rho_fn = getattr(o.lib, f"o{m.ndims}_mat_rho")
+    rho = np.array([rho_fn(C.byref(o.S), C.byref(o.params), e) for e in range(m.nelem)])
+    fz_weight = -np.sum(rho * grav * o.f["volume"])
+    assert np.max(np.abs(f[interior, :m.ndims - 1])) <= 1e-8 * typical
+    assert abs(f[:, m.ndims - 1].sum() - fz_weight) <= 1e-10 * abs(fz_weight)
+    assert np.max(np.abs(f[:, :m.ndims - 1].sum(axis=0))) <= 1e-8 * typical
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2"])
+def test_linear_temperature_is_steady(name):
+    """A linear temperature profile without heat sources does not change at
+    interior nodes (the lumped diffusion operator annihilates linear fields)."""
+    m = SC.build(name, nelem=1500)
+    m.params.radiogenic_heating[0] = 0.0
+    o = OracleState(m)
+    z = o.f["coord"][:, m.ndims - 1]
+    T = 273 + 1300 * (-z / m.lengths[-1])
+    o.f["temperature"][:] = T
+    o.call("update_temperature")
+    interior = m.bcflag == 0
+    assert np.max(np.abs(o.f["temperature"][interior] - T[interior])) <= 1e-9 * 1573
+
+
+@pytest.mark.parametrize("name", ["evp3", "mat2"])
+def test_mass_sums(name):
+    m = SC.build(name, nelem=1500)
+    o = OracleState(m)
+    V = o.f["volume"]
+    ps = m.params.max_vbc_val * m.params.inertial_scaling
+    K = m.bulkm_elem()
+    assert abs(o.f["mass"].sum() - np.sum(K / ps ** 2 * V)) <= 1e-12 * o.f["mass"].sum()
+    assert abs(o.f["volume_n"].sum() - m.npe * V.sum()) <= 1e-12 * o.f["volume_n"].sum()
+
+
+# ------------------------------------------------ velocity boundary conditions
+
+def _vbc_state(ndims, flags, types_, values, bnormals=None, edges=None):
+    n = len(flags)
+    coord = np.zeros((n, ndims))
+    conn = [list(range(ndims + 1))] if n >= ndims + 1 else [[0] * (ndims + 1)]
+    m = _tiny_model(coord if n >= ndims + 1 else np.zeros((ndims + 1, ndims)), conn, ndims=ndims)
+    m.bcflag = np.zeros(max(n, ndims + 1), dtype=np.uint32)
+    m.bcflag[:n] = flags
+    for i, (t, v) in enumerate(zip(types_, values)):
+        m.params.vbc_types[i] = t
+        m.params.vbc_values[i] = v
+    if bnormals is not None:
+        m.bnormals = np.ascontiguousarray(bnormals, dtype=np.float64)
+    if edges is not None:
+        ev, valid = edges
+        m.edge_vectors, m.edge_valid = ev, valid
+    return OracleState(m, init=False)
+
+
+@pytest.mark.parametrize("ndims", [2, 3])
+def test_vbc_meaning_of_wall_types(ndims):
+    """bc.vbc_x0 types (input.cxx help): 1 normal fixed; 2 shear fixed at 0;
+    3 normal fixed and shear 0; x walls act on x, z walls on the last axis."""
+    rng = np.random.default_rng(0)
+    v0 = rng.normal(0, 1, ndims)
+    for t, check in [(0, lambda v: np.array_equal(v, v0)),
+                     (1, lambda v: v[0] == 7.0 and np.array_equal(v[1:], v0[1:])),
+                     (2, lambda v: v[0] == v0[0] and np.all(v[1:] == 0)),
+                     (3, lambda v: v[0] == 7.0 and np.all(v[1:] == 0))]:
+        types_ = [t] + [0] * 9
+        o = _vbc_state(ndims, [M.BOUNDX0], types_, [7.0] + [0] * 9)
+        o.f["vel"][0] = v0
+        o.call("apply_vbcs")
+        assert check(o.f["vel"][0]), (t, o.f["vel"][0])
+    # bottom wall type 1 fixes the vertical component, and z is applied last
+    types_ = [1, 0, 0, 0, 1, 0, 0, 0, 0, 0]
+    o = _vbc_state(ndims, [M.BOUNDX0 | M.BOUNDZ0], types_, [7.0, 0, 0, 0, -2.0, 0, 0, 0, 0, 0])
+    o.f["vel"][0] = v0
+    o.call("apply_vbcs")
+    assert o.f["vel"][0][0] == 7.0 and o.f["vel"][0][ndims - 1] == -2.0
+
+
+def test_vbc_slant_boundary_sets_normal_velocity():
+    """Type 1 on a slant boundary sets v.n to the value and keeps the
+    tangential part; type 3 makes v parallel to n; at an intersection with a
+    type-1 wall the velocity is projected on the edge vector."""
+    n0 = np.array([0.6, 0.0, 0.8])
+    bn = np.zeros((10, 3))
+    bn[6] = n0
+    types_ = [1, 0, 0, 0, 0, 0, 1, 0, 0, 0]
+    vals = [0.0] * 6 + [0.5, 0, 0, 0]
+    o = _vbc_state(3, [1 << 6], types_, vals, bnormals=bn)
+    v0 = np.array([1.0, 2.0, 3.0])
+    o.f["vel"][0] = v0
+    o.call("apply_vbcs")
+    v = o.f["vel"][0]
+    assert abs(v @ n0 - 0.5) <= 1e-15
+    t0 = v0 - (v0 @ n0) * n0
+    assert np.allclose(v - (v @ n0) * n0, t0, rtol=0, atol=1e-15)
+    # type 3: v = value * n
+    types_[6] = 3
+    o = _vbc_state(3, [1 << 6], types_, vals, bnormals=bn)
+    o.f["vel"][0] = v0
+    o.call("apply_vbcs")
+    assert np.allclose(o.f["vel"][0], 0.5 * n0, rtol=0, atol=1e-16)
+    # intersection of x0 (type 1) and n0 (type 1): v parallel to the edge
+    types_[6] = 1
+    ev = np.zeros((10, 10, 3))
+    valid = np.zeros((10, 10), dtype=np.int32)
+    edge = np.array([0.0, 1.0, 0.0])
+    ev[0, 6] = edge
+    valid[0, 6] = 1
+    o = _vbc_state(3, [M.BOUNDX0 | (1 << 6)], types_, vals, bnormals=bn, edges=(ev, valid))
+    o.f["vel"][0] = v0
+    o.call("apply_vbcs")
+    v = o.f["vel"][0]
+    assert np.allclose(v, (v @ edge) * edge, rtol=0, atol=1e-15)
+    # a missing edge vector is reported, as the reference's map::at would throw
+    o = _vbc_state(3, [M.BOUNDX0 | (1 << 6)], types_, vals, bnormals=bn)
+    o.call("apply_vbcs")
+    assert o.S.status == abi.DES_ERR_ASSERT
