@@ -1061,13 +1061,16 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
         sd[k] = s.sdev[k * E + e];
         es[k] = s.strain32[k * E + e];
     }
+    float rs[NSTR], re[NSTR];
+#pragma unroll
+    for (int k = 0; k < NSTR; ++k) rs[k] = re[k] = 0;
     if (s.sc->rot_pending) {
         double w[3];
         spin<ND>(v, sh, w);
         const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
         const float dtp = (float)s.sc->dt_prev;
-        jaumann_t<ND, float>(sd, dtp, wf);
-        jaumann_t<ND, float>(es, dtp, wf);
+        jaumann_inc<ND, float>(sd, dtp, wf, rs);
+        jaumann_inc<ND, float>(es, dtp, wf, re);
     }
     {
         double sr[NSTR];
@@ -1083,7 +1086,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     double dp;
     const float Tm = (float)elem_mean<ND>(s.T, c);
     bool ok = stress_update_mixed<ND>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p, sd,
-                                      es, edot, pl, dpl, dp);
+                                      rs, es, re, edot, pl, dpl, dp);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     s.pres[e] = p;
 #pragma unroll

@@ -959,27 +959,23 @@ template <int ND> __device__ __forceinline__ float second_invariant2_f(const flo
     }
 }
 
-template <int ND, typename T> __device__ __forceinline__ void jaumann_t(T *s, T dt, const T w[3])
+// Jaumann increment dt (S W - W S) of a symmetric tensor (fields.cxx:375-404)
+template <int ND, typename T> __device__ __forceinline__ void jaumann_inc(const T *s, T dt, const T w[3],
+                                                                         T *inc)
 {
     if constexpr (ND == 3) {
         const T w3 = w[0], w4 = w[1], w5 = w[2];
-        T s_inc[6];
-        s_inc[0] = T(-2) * s[3] * w3 - T(2) * s[4] * w4;
-        s_inc[1] = T(2) * s[3] * w3 - T(2) * s[5] * w5;
-        s_inc[2] = T(2) * s[4] * w4 + T(2) * s[5] * w5;
-        s_inc[3] = s[0] * w3 - s[1] * w3 - s[4] * w5 - s[5] * w4;
-        s_inc[4] = s[0] * w4 - s[2] * w4 + s[3] * w5 - s[5] * w3;
-        s_inc[5] = s[1] * w5 - s[2] * w5 + s[3] * w4 + s[4] * w3;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) s[i] += dt * s_inc[i];
+        inc[0] = dt * (T(-2) * s[3] * w3 - T(2) * s[4] * w4);
+        inc[1] = dt * (T(2) * s[3] * w3 - T(2) * s[5] * w5);
+        inc[2] = dt * (T(2) * s[4] * w4 + T(2) * s[5] * w5);
+        inc[3] = dt * (s[0] * w3 - s[1] * w3 - s[4] * w5 - s[5] * w4);
+        inc[4] = dt * (s[0] * w4 - s[2] * w4 + s[3] * w5 - s[5] * w3);
+        inc[5] = dt * (s[1] * w5 - s[2] * w5 + s[3] * w4 + s[4] * w3);
     } else {
         const T w2 = w[0];
-        T s_inc[3];
-        s_inc[0] = T(-2) * s[2] * w2;
-        s_inc[1] = T(2) * s[2] * w2;
-        s_inc[2] = s[0] * w2 - s[1] * w2;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) s[i] += dt * s_inc[i];
+        inc[0] = dt * (T(-2) * s[2] * w2);
+        inc[1] = dt * (T(2) * s[2] * w2);
+        inc[2] = dt * (s[0] * w2 - s[1] * w2);
     }
 }
 
@@ -1050,32 +1046,39 @@ __device__ __forceinline__ void mat_plastic_props_f(const DevState &s, int e, fl
     ten_max = (phi == 0) ? P->tension_max : dmin(P->tension_max, (double)(cohesion / tanf(phi * deg)));
 }
 
-// elastic increment of (p, sd): the diagonal mean takes (lambda + 2G/ND) tr(de)
-template <int ND> __device__ __forceinline__ void elastic_split(float K, float G, const float *de,
-                                                               double &p, float *sd)
+// The FP32 deviator is only ever changed by adding one increment per step:
+// the Jaumann rotation rs (~1e-8 of the stress, below an FP32 ulp on its own)
+// and the rheological increment are summed first and rounded once.
+
+// elastic increment (rheology.cxx:105-115) of (p, sd): the diagonal mean
+// takes (lambda + 2G/ND) tr(de), the deviator 2G (de - tr(de)/ND)
+template <int ND> __device__ __forceinline__ void elastic_incr(float K, float G, const float *de,
+                                                              const float *rs, double &p, float *inc)
 {
     const float lambda = K - 2.f / 3 * G;
     const float tr = trace_f<ND>(de), mean = tr / ND;
 #pragma unroll
-    for (int i = 0; i < ND; ++i) sd[i] += 2 * G * (de[i] - mean);
+    for (int i = 0; i < ND; ++i) inc[i] = rs[i] + 2 * G * (de[i] - mean);
 #pragma unroll
-    for (int i = ND; i < Dim<ND>::NSTR; ++i) sd[i] += 2 * G * de[i];
+    for (int i = ND; i < Dim<ND>::NSTR; ++i) inc[i] = rs[i] + 2 * G * de[i];
     p += (double)((lambda + 2 * G / ND) * tr);
 }
 
-// maxwell (rheology.cxx:118-134) on (p, sd): the mean of the diagonal only
-// takes the bulk term
-template <int ND> __device__ __forceinline__ void maxwell_split(float K, float G, float eta, float dt,
-                                                               double dv, const float *de, double &p,
-                                                               float *sd)
+// maxwell (rheology.cxx:118-134) in increment form: with t = dt G / (2 eta),
+// s' = (s (1 - t) + 2G de') / (1 + t) = s + (2G de' - 2t s) / (1 + t); the
+// relaxation 2t s keeps FP32's relative precision where 1 - t would round t
+// to ~1e-3 of itself.  The diagonal mean takes only the bulk term.
+template <int ND> __device__ __forceinline__ void maxwell_incr(float K, float G, float eta, float dt,
+                                                              double dv, const float *de, const float *sd,
+                                                              const float *rs, double &p, float *inc)
 {
-    float tmp = 0.5f * dt * G / eta;
-    float f1 = 1 - tmp, f2 = 1 / (1 + tmp);
-    float dev = trace_f<ND>(de) / ND;
+    const float t = 0.5f * dt * G / eta;
+    const float f2 = 1 / (1 + t);
+    const float dev = trace_f<ND>(de) / ND;
 #pragma unroll
-    for (int i = 0; i < ND; ++i) sd[i] = (sd[i] * f1 + 2 * G * (de[i] - dev)) * f2;
+    for (int i = 0; i < ND; ++i) inc[i] = rs[i] + (2 * G * (de[i] - dev) - 2 * t * sd[i] - 2 * t * rs[i]) * f2;
 #pragma unroll
-    for (int i = ND; i < Dim<ND>::NSTR; ++i) sd[i] = (sd[i] * f1 + 2 * G * de[i]) * f2;
+    for (int i = ND; i < Dim<ND>::NSTR; ++i) inc[i] = rs[i] + (2 * G * de[i] - 2 * t * sd[i] - 2 * t * rs[i]) * f2;
     p += (double)K * dv;
 }
 
@@ -1096,11 +1099,14 @@ template <int ND> __device__ __forceinline__ void split_stress(const double *s, 
     for (int i = ND; i < Dim<ND>::NSTR; ++i) sd[i] = (float)s[i];
 }
 
-// update_stress for one element in mixed precision (plane strain excluded)
+// update_stress for one element in mixed precision (plane strain excluded).
+// rs / re: Jaumann increments of the deviator and the strain from the
+// previous step's rotation, folded into this step's single rounding.
 template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf,
                                                                      float edvoldt, float Tmean, double vol,
                                                                      double vol_old, double &p, float *sd,
-                                                                     float *es, float *edot, float &pl,
+                                                                     const float *rs, float *es,
+                                                                     const float *re, float *edot, float &pl,
                                                                      float &dpl, double &dpressure)
 {
     constexpr int NSTR = Dim<ND>::NSTR;
@@ -1111,17 +1117,21 @@ template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevS
 #pragma unroll
         for (int i = 0; i < ND; ++i) edot[i] += (edvoldt - div) / ND;
     }
-#pragma unroll
-    for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dtf;
     float de[NSTR];
 #pragma unroll
-    for (int i = 0; i < NSTR; ++i) de[i] = edot[i] * dtf;
+    for (int i = 0; i < NSTR; ++i) {
+        de[i] = edot[i] * dtf;
+        es[i] += re[i] + de[i];
+    }
     const float K = (float)harmonic_mean(st, P->bulk_modulus, e);
     const float G = (float)harmonic_mean(st, P->shear_modulus, e);
+    float inc[NSTR];
     bool ok = true;
     switch (P->rheol_type) {
     case DES_RH_ELASTIC:
-        elastic_split<ND>(K, G, de, p, sd);
+        elastic_incr<ND>(K, G, de, rs, p, inc);
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) sd[i] += inc[i];
         break;
     case DES_RH_VISCOUS: {
         float eta = mat_visc_f<ND>(st, e, Tmean, edot);
@@ -1135,36 +1145,45 @@ template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevS
     }
     case DES_RH_MAXWELL: {
         float eta = mat_visc_f<ND>(st, e, Tmean, edot);
-        maxwell_split<ND>(K, G, eta, dtf, vol / vol_old - 1, de, p, sd);
+        maxwell_incr<ND>(K, G, eta, dtf, vol / vol_old - 1, de, sd, rs, p, inc);
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) sd[i] += inc[i];
         break;
     }
     case DES_RH_EP:
     case DES_RH_EVP: {
         double pv = p;
-        float sv[NSTR];
+        float incv[NSTR];
         float svII = 0;
         if (P->rheol_type == DES_RH_EVP) {
-#pragma unroll
-            for (int i = 0; i < NSTR; ++i) sv[i] = sd[i];
             float eta = mat_visc_f<ND>(st, e, Tmean, edot);
-            maxwell_split<ND>(K, G, eta, dtf, vol / vol_old - 1, de, pv, sv);
+            maxwell_incr<ND>(K, G, eta, dtf, vol / vol_old - 1, de, sd, rs, pv, incv);
+            float sv[NSTR];
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) sv[i] = sd[i] + incv[i];
             svII = second_invariant2_f<ND>(sv);
         }
         double amc, anphi, anpsi, hardn, ten_max;
         mat_plastic_props_f(st, e, pl, amc, anphi, anpsi, hardn, ten_max);
         double pp = p;
-        float sp[NSTR];
-#pragma unroll
-        for (int i = 0; i < NSTR; ++i) sp[i] = sd[i];
-        elastic_split<ND>(K, G, de, pp, sp);
+        elastic_incr<ND>(K, G, de, rs, pp, inc);
         double s64[NSTR];
-        join_stress<ND>(pp, sp, s64);
+#pragma unroll
+        for (int i = 0; i < ND; ++i) s64[i] = (double)sd[i] + (double)inc[i] + pp;
+#pragma unroll
+        for (int i = ND; i < NSTR; ++i) s64[i] = (double)sd[i] + (double)inc[i];
         double depls;
         mc_return<ND>((double)K, (double)G, amc, anphi, anpsi, hardn, ten_max, depls, s64);
         if (P->rheol_type == DES_RH_EVP && (double)svII < second_invariant2<ND>(s64)) {
             p = pv;
 #pragma unroll
-            for (int i = 0; i < NSTR; ++i) sd[i] = sv[i];
+            for (int i = 0; i < NSTR; ++i) sd[i] += incv[i];
+        } else if (depls == 0) {
+            // elastic trial accepted: keep the single-rounding increment
+            p = pp;
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) sd[i] += inc[i];
+            dpl = 0;
         } else {
             split_stress<ND>(s64, p, sd);
             pl += (float)depls;

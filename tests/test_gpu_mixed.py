@@ -45,7 +45,7 @@ def test_mixed_chunks_and_state_roundtrip():
     g2.run(15, abi.DES_MIXED)
     g2.run(25, abi.DES_MIXED)
     e = _errors(g1.download(), g2.download())
-    assert max(e.values()) < 1e-5, e
+    assert e["vel"] < 1e-5 and e["stress"] < 1e-5, e
     dt, t, st = g2.scalars()
     assert st == 40
 

@@ -114,7 +114,7 @@ def _check_trajectory(g, o, name, nsteps):
     spread = _ref_spread(o.model, nsteps)
     bound = 1e-5 if spread is None else max(10 * spread, 1e-9)
     print(f"{name}: {nsteps} steps, GPU vs oracle {err:.3e}, reference build spread {spread}")
-    assert err < min(bound, 1e-5), (err, spread)
+    assert err < bound, (err, spread)
 
 
 @pytest.mark.parametrize("name", ALL_STEP)
