@@ -1,14 +1,16 @@
 """Benchmark of the explicit time step on one MI355X.
 
-Workload: the 3D FP64 extension box -- 100 x 100 x 10 km, ~1.04 M tetrahedra /
-190 k nodes, elasto-visco-plastic Mohr-Coulomb (cohesion 44 -> 4 MPa, friction
-30 -> 5 deg over plastic strain 0.1), linear 273 -> 1573 K temperature, 5 km
-weak slab plus seeded plastic-strain noise, +-1 cm/yr side walls,
-dynamic-relaxation mass scaling.  Synthetic inputs built on the host, resident
-in HBM before the timed region.
+Default workload (--workload 3d-fp64): the 3D FP64 extension box -- 100 x 100 x
+10 km, ~1.02 M tetrahedra / 187 k nodes, elasto-visco-plastic Mohr-Coulomb
+(cohesion 44 -> 4 MPa, friction 30 -> 5 deg over plastic strain 0.1), linear
+273 -> 1573 K temperature, 5 km weak slab plus seeded plastic-strain noise,
++-1 cm/yr side walls, dynamic-relaxation mass scaling.
+--workload 3d-mixed: 200 x 200 x 20 km, ~8.1 M tetrahedra / 1.4 M nodes,
+per-element log-viscosity U[21,25] and cohesion N(44, 4.4) MPa.
 
-A step is one pass of the explicit step over the whole mesh through the fused
-six-pass schedule (des_run).  Prints one JSON line.
+Synthetic inputs are built on the host and resident in HBM before the timed
+region.  A step is one pass of the explicit step over the whole mesh through
+the fused six-pass schedule (des_run).  Prints one JSON line.
 """
 import argparse
 import json
@@ -16,19 +18,28 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "elem-steps/s & % HBM peak: FP64 vs mixed-prec step; adjoint/forward time"
 
-# Algorithmic bytes of the fused passes (3D FP64), per element and per node:
+# Algorithmic bytes of the fused passes per element and per node (3D, FP64):
 # pass C reads conn, volume, volume_old, stress, strain, plstrain and writes
 # stress, strain, plstrain, delta_plstrain, dpressure (256 B/element) and
 # gathers coordinates, velocity, dvoldt and temperature (64 B/node).
 PASS_BYTES_3D = {
     "pass_a": (72, 56), "pass_b": (72, 48), "pass_c": (256, 64),
     "pass_d": (32, 20), "pass_e": (200, 40), "pass_f": (112, 112),
+}
+# mixed precision: FP32 deviator/strain/plastic strain in passes C and E
+PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
+
+WORKLOADS = {
+    "3d-fp64": ("3d-fp64 extension box 100x100x10 km, EVP Mohr-Coulomb", 1_000_000),
+    "3d-mixed": ("3d-mixed box 200x200x20 km, per-element viscosity and cohesion, EVP", 8_000_000),
 }
 
 
@@ -55,54 +66,84 @@ def cpu_baseline(model, nsteps):
                       f"{secs:.2f} s"}
 
 
+def timed(g, steps, precision):
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(steps, precision)
+    g.sync()
+    return time.perf_counter() - t0
+
+
+def rel(a, b):
+    s = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a - b))) / s if s else float(np.max(np.abs(a - b)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--nelem", type=int, default=1_000_000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d-fp64")
+    ap.add_argument("--nelem", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
               "not part of this build", file=sys.stderr)
         sys.exit(2)
 
-    from dynearthsol_amd import model as M
+    from dynearthsol_amd import abi, model as M
     from dynearthsol_amd.lib import DesModel
 
-    m = M.config_benchmark("3d-fp64", nelem_target=args.nelem)
-    g = DesModel(m)
+    desc, ne_default = WORKLOADS[args.workload]
+    nelem = args.nelem or ne_default
+    primary = abi.DES_MIXED if args.workload == "3d-mixed" else abi.DES_FP64
+    m = M.config_benchmark(args.workload, nelem_target=nelem)
     E, N = m.nelem, m.nnode
-    print(f"# model built: {E} elements, {N} nodes; warmup {args.warmup}", file=sys.stderr, flush=True)
-    g.run(args.warmup)
-    g.sync()
-
-    t0 = time.perf_counter()
-    g.run(args.steps)
-    g.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
+    g = DesModel(m)
+    print(f"# {args.workload}: {E} elements, {N} nodes; warmup {args.warmup}", file=sys.stderr, flush=True)
+    g.run(args.warmup, primary)
+    secs = timed(g, args.steps, primary)
     ms_per_step = 1e3 * secs / args.steps
     value = E * args.steps / secs
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
 
-    # per-pass durations with HIP events on the library's stream
-    prof_steps = min(args.steps, 500)
-    passes = g.run_timed(prof_steps)
+    # per-pass durations with HIP events on the library's stream, same number of steps
+    passes = g.run_timed(args.steps, primary)
     g.sync()
     avg = {k: ms / max(c, 1) for k, (ms, c) in passes.items()}
-    dom = max((k for k in avg if k in PASS_BYTES_3D), key=lambda k: avg[k])
-    be, bn = PASS_BYTES_3D[dom]
-    traffic_alg = be * E + bn * N
-    achieved = traffic_alg / (avg[dom] * 1e-3) / 1e9
-    step_bytes = sum(be_ * E + bn_ * N for be_, bn_ in PASS_BYTES_3D.values())
+    table = PASS_BYTES_3D_MIXED if primary == abi.DES_MIXED else PASS_BYTES_3D
+    dom = max((k for k in avg if k in table), key=lambda k: avg[k])
+    be, bn = table[dom]
+    achieved = (be * E + bn * N) / (avg[dom] * 1e-3) / 1e9
+    step_bytes = sum(be_ * E + bn_ * N for be_, bn_ in table.values())
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+
+    # FP64 and mixed precision side by side from the same initial state
+    compare = None
+    if not args.no_mixed:
+        other = abi.DES_FP64 if primary == abi.DES_MIXED else abi.DES_MIXED
+        g1, g2 = DesModel(m), DesModel(m)
+        n_cmp = min(args.steps, 2000)
+        g1.run(args.warmup, abi.DES_FP64)
+        g2.run(args.warmup, abi.DES_MIXED)
+        t64 = timed(g1, n_cmp, abi.DES_FP64)
+        t32 = timed(g2, n_cmp, abi.DES_MIXED)
+        a, b = g2.download(["vel", "stress", "coord", "plstrain"]), g1.download(["vel", "stress", "coord", "plstrain"])
+        compare = {"steps_after_warmup": n_cmp + args.warmup,
+                   "fp64_ms_per_step": 1e3 * t64 / n_cmp, "mixed_ms_per_step": 1e3 * t32 / n_cmp,
+                   "mixed_speedup": t64 / t32,
+                   "mixed_rel_err_vs_fp64": {k: rel(a[k], b[k]) for k in a}}
+        g1.close()
+        g2.close()
+        del other
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(M.config_benchmark("3d-fp64", nelem_target=args.nelem), args.cpu_steps)
+        cpu = cpu_baseline(M.config_benchmark(args.workload, nelem_target=nelem), args.cpu_steps)
 
     out = {
         "metric": METRIC,
@@ -115,15 +156,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if primary == abi.DES_FP64 else "f32 deviator/rheology + f64 pressure/geometry/force",
         "data": "synthetic",
-        "config": {"workload": "3d-fp64 extension box 100x100x10 km, EVP Mohr-Coulomb",
-                   "nelem": E, "nnode": N, "schedule": "fused six-pass", "precision": "fp64"},
+        "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
+                   "precision": "fp64" if primary == abi.DES_FP64 else "mixed"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "step_hbm": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},
+        "fp64_vs_mixed": compare,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -534,6 +534,63 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_velocity(Dev
     for (int d = 0; d < ND; ++d) s.vel[d * s.N + n] += dt * s.force[d * s.N + n] / m;
 }
 
+struct Vec3 {
+    double v[3];
+    int ok;
+};
+
+// the slant boundaries n0..n3 of apply_vbcs (bc.cxx:350-436); out of line and
+// by value because no box model has them
+template <int ND> __device__ __noinline__ Vec3 vbc_slant(const DevState &s, unsigned flag, Vec3 in)
+{
+    const int *T = s.P->vbc_types;
+    const double *V = s.P->vbc_values;
+    double *v = in.v;
+    for (int ib = IBOUNDN0; ib <= IBOUNDN3; ib++) {
+        if (!(flag & (1u << ib))) continue;
+        const double *nv = s.bnormals + ib * ND;
+        double fac = 0;
+        switch (T[ib]) {
+        case 1:
+        case 11: {
+            const int nh = (T[ib] == 1) ? ND : ND - 1;   // type 11: horizontal components only
+            if (T[ib] == 11) fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
+            const double target = (T[ib] == 1) ? V[ib] : V[ib] * fac;
+            if (flag == (1u << ib)) {
+                double vn = 0;
+                for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
+                for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
+            } else {
+                for (int ic = 0; ic < ib; ic++) {
+                    if (!(flag & (1u << ic))) continue;
+                    if (T[ic] == 0) {
+                        double vn = 0;
+                        for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
+                        for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
+                    } else if (T[ic] == 1) {
+                        if (!s.edge_valid[ic * DES_NBDRY + ib]) { in.ok = 0; continue; }
+                        const double *edge = s.edge_vectors + (ic * DES_NBDRY + ib) * ND;
+                        double ve = 0;
+                        for (int d = 0; d < ND; d++) ve += v[d] * edge[d];
+                        for (int d = 0; d < ND; d++) v[d] = ve * edge[d];
+                    }
+                }
+            }
+            break;
+        }
+        case 3:
+            for (int d = 0; d < ND; d++) v[d] = V[ib] * nv[d];
+            break;
+        case 13:
+            fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
+            for (int d = 0; d < ND - 1; d++) v[d] = V[ib] * fac * nv[d];
+            v[ND - 1] = 0;
+            break;
+        }
+    }
+    return in;
+}
+
 // apply_vbcs for one node (bc.cxx:179-490); returns false if an edge vector
 // the reference's map lacks is needed
 template <int ND> __device__ __forceinline__ bool vbc_node(const DevState &s, unsigned flag, double v[ND])
@@ -584,47 +641,15 @@ template <int ND> __device__ __forceinline__ bool vbc_node(const DevState &s, un
             }
         }
     }
-    for (int ib = IBOUNDN0; ib <= IBOUNDN3; ib++) {
-        if (!(flag & (1u << ib))) continue;
-        const double *nv = s.bnormals + ib * ND;
-        double fac = 0;
-        switch (T[ib]) {
-        case 1:
-        case 11: {
-            const int nh = (T[ib] == 1) ? ND : ND - 1;   // type 11: horizontal components only
-            if (T[ib] == 11) fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
-            const double target = (T[ib] == 1) ? V[ib] : V[ib] * fac;
-            if (flag == (1u << ib)) {
-                double vn = 0;
-                for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
-                for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
-            } else {
-                for (int ic = 0; ic < ib; ic++) {
-                    if (!(flag & (1u << ic))) continue;
-                    if (T[ic] == 0) {
-                        double vn = 0;
-                        for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
-                        for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
-                    } else if (T[ic] == 1) {
-                        if (!s.edge_valid[ic * DES_NBDRY + ib]) { ok = false; continue; }
-                        const double *edge = s.edge_vectors + (ic * DES_NBDRY + ib) * ND;
-                        double ve = 0;
-                        for (int d = 0; d < ND; d++) ve += v[d] * edge[d];
-                        for (int d = 0; d < ND; d++) v[d] = ve * edge[d];
-                    }
-                }
-            }
-            break;
-        }
-        case 3:
-            for (int d = 0; d < ND; d++) v[d] = V[ib] * nv[d];
-            break;
-        case 13:
-            fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
-            for (int d = 0; d < ND - 1; d++) v[d] = V[ib] * fac * nv[d];
-            v[ND - 1] = 0;
-            break;
-        }
+    if (flag & 0x3C0u) {     // BOUNDN0..BOUNDN3
+        Vec3 in;
+        in.ok = 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) in.v[d] = d < ND ? v[d] : 0.0;
+        Vec3 out = vbc_slant<ND>(s, flag, in);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) v[d] = out.v[d];
+        if (!out.ok) ok = false;
     }
     if (T[4] == 0 && T[5] == 0) return ok;
     if (flag & BOUNDZ0) {
@@ -830,12 +855,28 @@ struct FusedFlags {
     int store_aux;         // last step before a sync: keep strain_rate, edvoldt, force
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumbering them so that XCD x runs one contiguous range of logical
+// blocks gives every XCD the same contiguous slab of the (renumbered) mesh in
+// every pass: what one pass writes is re-read from the same L2 by the next.
+__device__ __forceinline__ int xcd_block()
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// Packed per-element record written by pass A and gathered by pass B: the
+// thermal contributions to the element's nodes, tr(edot) V, the element's
+// mass and thermal mass and its volume, one 64-byte line per element.
+constexpr int REC = 8;
+constexpr int REC_DJ = 4, REC_M = 5, REC_TM = 6, REC_V = 7;
+
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
+    int e = xcd_block() * BLOCK + threadIdx.x;
     if (e >= s.E) return;
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
-    const long E = s.E;
     const des_params *P = s.P;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -844,48 +885,50 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, 
     double vol = elem_volume<ND>(x);
     shape_fn<ND>(x, vol, sh);
     s.volume[e] = vol;
+    double rec[REC];
+#pragma unroll
+    for (int k = 0; k < REC; ++k) rec[k] = 0;
     double Tn[NPE];
 #pragma unroll
     for (int i = 0; i < NPE; ++i) Tn[i] = s.T[c[i]];
-    if (P->has_thermal_diffusion) {
-        double out[NPE];
-        temperature_contrib<ND>(s, e, vol, sh, Tn, out);
-#pragma unroll
-        for (int i = 0; i < NPE; ++i) s.ework[e * NPE + i] = out[i];
-    }
+    if (P->has_thermal_diffusion) temperature_contrib<ND>(s, e, vol, sh, Tn, rec);
     gather_vec<ND>(s.vel, s.N, c, v);
     double sr[NSTR];
     strain_rate<ND>(v, sh, sr);
-    s.escal[0 * E + e] = trace<ND>(sr) * vol;
+    rec[REC_DJ] = trace<ND>(sr) * vol;
     if (!fl.use_stored_mass) {
         double Tm = 0;
 #pragma unroll
         for (int i = 0; i < NPE; ++i) Tm += Tn[i];
         Tm /= NPE;
-        double m, tm;
-        mass_of<ND>(s, e, vol, Tm, m, tm);
-        s.escal[1 * E + e] = m;
-        s.escal[2 * E + e] = tm;
+        mass_of<ND>(s, e, vol, Tm, rec[REC_M], rec[REC_TM]);
+        rec[REC_V] = vol;
     }
+    double2 *out = reinterpret_cast<double2 *>(s.erec + (long)e * REC);
+#pragma unroll
+    for (int k = 0; k < REC / 2; ++k) out[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
 }
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
+    int n = xcd_block() * BLOCK + threadIdx.x;
     if (n >= s.N) return;
     constexpr int NPE = Dim<ND>::NPE;
-    const long E = s.E;
     const int thermal = s.P->has_thermal_diffusion;
     double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
-    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
+    const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
+    for (int p = p0; p < p1; ++p) {
         const int ent = s.csr_ent[p];
-        const int e = ent / NPE;
-        if (thermal) tdot += s.ework[ent];
-        dj += s.escal[e];
+        const int e = ent / NPE, i = ent - e * NPE;
+        const double *r = s.erec + (long)e * REC;
+        const double2 djm = *reinterpret_cast<const double2 *>(r + REC_DJ);
+        const double2 tmv = *reinterpret_cast<const double2 *>(r + REC_TM);
+        if (thermal) tdot += r[i];
+        dj += djm.x;
         if (!fl.use_stored_mass) {
-            vn += s.volume[e];
-            m += s.escal[1 * E + e];
-            if (thermal) tm += s.escal[2 * E + e];
+            vn += tmv.y;
+            m += djm.y;
+            if (thermal) tm += tmv.x;
         }
     }
     if (fl.use_stored_mass) {
@@ -901,54 +944,86 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
     s.ntmp[n] = dj / vn;
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c(DevState s, FusedFlags fl)
+template <int ND> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl, int e)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     int c[NPE];
     load_conn<ND>(s, e, c);
-    double x[NPE][ND], sh[ND][NPE], v[NPE][ND];
-    gather_vec<ND>(s.coord, s.N, c, x);
-    const double vol = s.volume[e];
-    shape_fn<ND>(x, vol, sh);
-    gather_vec<ND>(s.vel, s.N, c, v);
-    double st[NSTR], es[NSTR], edot[NSTR];
-    load_tensor<ND>(s.stress, E, e, st);
-    load_tensor<ND>(s.strain, E, e, es);
-    if (s.sc->rot_pending) {
-        double w[3];
-        spin<ND>(v, sh, w);
-        const double dtp = s.sc->dt_prev;
-        jaumann<ND>(st, dtp, w);
-        jaumann<ND>(es, dtp, w);
+    double sh[ND][NPE], v[NPE][ND];
+    {
+        double x[NPE][ND];
+        gather_vec<ND>(s.coord, s.N, c, x);
+        shape_fn<ND>(x, s.volume[e], sh);
     }
+    gather_vec<ND>(s.vel, s.N, c, v);
+    double w[3] = {0, 0, 0};
+    const bool rot = s.sc->rot_pending;
+    if (rot) spin<ND>(v, sh, w);
+    const double dtp = s.sc->dt_prev;
+    double edot[NSTR];
     strain_rate<ND>(v, sh, edot);
     double edv = 0;
 #pragma unroll
     for (int i = 0; i < NPE; ++i) edv += s.ntmp[c[i]];
     edv = edv / NPE;
-    double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
-    double Tm = elem_mean<ND>(s.T, c);
-    bool ok = stress_update<ND>(s, e, s.sc->dt, edv, Tm, vol, s.volume_old[e], st, syy, es, edot,
-                                pl, dpl, dp);
-    if (!ok) set_error(s.sc, DES_ERR_INPUT);
-    store_tensor<ND>(s.stress, E, e, st);
-    store_tensor<ND>(s.strain, E, e, es);
-    if (s.P->is_plane_strain) s.stressyy[e] = syy;
-    s.plstrain[e] = pl;
-    s.delta_plstrain[e] = dpl;
-    s.dpressure[e] = dp;
+    const double dt = s.sc->dt;
+    // anti-locking and the strain update of update_stress (rheology.cxx:528-540)
+    // first, so the strain is stored before the rheology needs the registers
+    double total_dv;
+    {
+        double div = trace<ND>(edot);
+#pragma unroll
+        for (int i = 0; i < ND; ++i) edot[i] += (edv - div) / ND;
+        double es[NSTR];
+        load_tensor<ND>(s.strain, E, e, es);
+        if (rot) jaumann<ND>(es, dtp, w);
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dt;
+        store_tensor<ND>(s.strain, E, e, es);
+        total_dv = trace<ND>(es);
+    }
     if (fl.store_aux) {
         store_tensor<ND>(s.strain_rate, E, e, edot);
         s.edvoldt[e] = edv;
     }
+    double st[NSTR];
+    load_tensor<ND>(s.stress, E, e, st);
+    if (rot) jaumann<ND>(st, dtp, w);
+    double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
+    const double vol = s.volume[e];
+    bool ok = rheology_after_strain<ND>(s, e, dt, total_dv, elem_mean<ND>(s.T, c), vol, s.volume_old[e], st,
+                                        syy, edot, pl, dpl, dp);
+    if (!ok) set_error(s.sc, DES_ERR_INPUT);
+    store_tensor<ND>(s.stress, E, e, st);
+    if (s.P->is_plane_strain) s.stressyy[e] = syy;
+    s.plstrain[e] = pl;
+    s.delta_plstrain[e] = dpl;
+    s.dpressure[e] = dp;
+    s.escal[e] = dp * vol;          // NMD_stress's dp * volume (geometry.cxx:206)
+}
+
+template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
+{
+    int e = xcd_block() * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    pass_c_body<ND>(s, fl, e);
+}
+
+// pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s)
+{
+    int n = xcd_block() * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    constexpr int NPE = Dim<ND>::NPE;
+    double sum = 0;
+    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) sum += s.escal[s.csr_ent[p] / NPE];
+    s.ntmp[n] = sum / s.volume_n[n];
 }
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
+    int e = xcd_block() * BLOCK + threadIdx.x;
     if (e >= s.E) return;
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
@@ -969,7 +1044,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, FusedFlags fl)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
+    int n = xcd_block() * BLOCK + threadIdx.x;
     if (n >= s.N) return;
     const double dt = s.sc->dt;
     double f[ND], v[ND];
@@ -1037,7 +1112,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevState s, FusedFlags fl)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
+    int e = xcd_block() * BLOCK + threadIdx.x;
     if (e >= s.E) return;
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
@@ -1097,6 +1172,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     s.pl32[e] = pl;
     s.dpl32[e] = dpl;
     s.dpressure[e] = dp;
+    s.escal[e] = dp * vol;
     if (fl.store_aux) {
 #pragma unroll
         for (int k = 0; k < NSTR; ++k) s.strain_rate[k * E + e] = (double)edot[k];
@@ -1106,7 +1182,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
+    int e = xcd_block() * BLOCK + threadIdx.x;
     if (e >= s.E) return;
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
@@ -1173,6 +1249,8 @@ struct des_ctx {
     double *elem0 = nullptr;       // bulkm(0), rho(0)
     int steps = 0;                 // host mirror of var.steps
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
+    int passc_variant = -1;        // waves-per-EU variant of pass C chosen by the autotuner
+    double passc_ms[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -1499,6 +1577,7 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
     ALLOC(s.shp, (size_t)E * npe * nd);
     ALLOC(s.ework, (size_t)E * npe * nd);
     ALLOC(s.escal, (size_t)E * 4);
+    ALLOC(s.erec, (size_t)E * 8);
     ALLOC(s.sc, 1);
     ALLOC(s.red, RED_BLOCKS * 3);
     ALLOC(s.redi, RED_BLOCKS);
@@ -1792,6 +1871,34 @@ int ensure_mixed(des_ctx *ctx)
     return DES_OK;
 }
 
+// Pass C with a minimum of 1, 2, 3 or 4 waves per SIMD: fewer registers per
+// lane buy occupancy at the price of spills.  All variants perform the same
+// arithmetic; the autotuner below keeps the fastest.
+constexpr int PASSC_VARIANTS = 4;
+
+int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
+{
+    dim3 g(nblocks(ctx->E)), b(BLOCK);
+#define PC(NDV, W) hipLaunchKernelGGL((k_pass_c<NDV, W>), g, b, 0, ctx->stream, ctx->s, fl)
+    if (ctx->nd == 3) {
+        switch (variant) {
+        case 0: PC(3, 1); break;
+        case 1: PC(3, 2); break;
+        case 2: PC(3, 3); break;
+        default: PC(3, 4); break;
+        }
+    } else {
+        switch (variant) {
+        case 0: PC(2, 1); break;
+        case 1: PC(2, 2); break;
+        case 2: PC(2, 3); break;
+        default: PC(2, 4); break;
+        }
+    }
+#undef PC
+    return check_launch(ctx);
+}
+
 int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
 {
     if (precision != DES_FP64 && precision != DES_MIXED)
@@ -1807,6 +1914,7 @@ int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
         LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);
     }
     const int rotate = (ctx->P.rheol_type & DES_RH_ELASTIC) ? 1 : 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tune;
     MARK(-1);
     for (int k = 0; k < nsteps; ++k) {
         FusedFlags fl;
@@ -1817,10 +1925,40 @@ int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
         MARK(0);
         LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
         MARK(1);
-        if (mixed) LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
-        else LAUNCH_ND(k_pass_c, nblocks(ctx->E), ctx->s, fl);
+        if (mixed) {
+            LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
+        } else if (ctx->passc_variant < 0 && tune.size() < 2 * PASSC_VARIANTS && nsteps >= 16) {
+            // autotuning: the first 2 x 4 steps of a long enough run time each variant twice
+            const int var = (int)tune.size() % PASSC_VARIANTS;
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            HIPCHK(hipEventRecord(a, ctx->stream));
+            if ((rc = launch_pass_c(ctx, fl, var))) return rc;
+            HIPCHK(hipEventRecord(b, ctx->stream));
+            tune.push_back({a, b});
+            if (tune.size() == 2 * PASSC_VARIANTS) {
+                HIPCHK(hipEventSynchronize(b));
+                double best = 1e300;
+                for (int v = 0; v < PASSC_VARIANTS; ++v) ctx->passc_ms[v] = 0;
+                for (size_t k = 0; k < tune.size(); ++k) {
+                    float t = 0;
+                    HIPCHK(hipEventElapsedTime(&t, tune[k].first, tune[k].second));
+                    ctx->passc_ms[k % PASSC_VARIANTS] += t;
+                    (void)hipEventDestroy(tune[k].first);
+                    (void)hipEventDestroy(tune[k].second);
+                }
+                for (int v = 0; v < PASSC_VARIANTS; ++v)
+                    if (ctx->passc_ms[v] < best) {
+                        best = ctx->passc_ms[v];
+                        ctx->passc_variant = v;
+                    }
+            }
+        } else {
+            if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
+        }
         MARK(2);
-        LAUNCH_ND(k_nmd_node, nblocks(ctx->N), ctx->s);
+        LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s);
         MARK(3);
         if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);

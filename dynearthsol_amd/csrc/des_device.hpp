@@ -61,6 +61,7 @@ struct DevState {
     double *shp;                                  // ND x NPE x E
     const double *elem_visc, *elem_cohesion;      // E or null
     double *ework;                                // E x NPE x ND contributions
+    double *erec;                                 // E x 8 packed records (fused pass A -> B)
     double *escal;                                // 4 x E element scalars
     double *ftrac;                                // nbf x ND facet tractions
     DevScalars *sc;
@@ -368,6 +369,9 @@ template <int ND> __device__ __forceinline__ double mat_visc(const DevState &s, 
     double visc;
     if (s.elem_visc) {
         visc = s.elem_visc[e];
+    } else if (P->visc_min == P->visc_max) {
+        // the clamp below maps every non-NaN power-law value to this one
+        return P->visc_min;
     } else {
         const double gas_constant = 8.3144;
         const double min_strain_rate = 1e-30;
@@ -841,25 +845,21 @@ template <int ND> __device__ __forceinline__ void plastic_return(double bulkm, d
     elasto_plastic<ND>(bulkm, shearm, amc, anphi, anpsi, hardn, ten_max, de, depls, s);
 }
 
-// Body of update_stress for one element (rheology.cxx:520-641).  edot is
-// anti-locked in place; returns false for an unknown rheology.
-template <int ND> __device__ __forceinline__ bool stress_update(const DevState &st, int e, double dt,
-                                                               double edvoldt, double Tmean,
-                                                               double vol, double vol_old,
-                                                               double *s, double &syy, double *es,
-                                                               double *edot, double &plstrain,
-                                                               double &delta_plstrain, double &dpressure)
+// The rheology part of update_stress (rheology.cxx:542-641) for one element
+// whose strain rate is already anti-locked and whose strain is already
+// updated; total_dv = trace(strain) for the viscous rheology.  Returns false
+// for an unknown rheology.
+template <int ND> __device__ __forceinline__ bool rheology_after_strain(const DevState &st, int e, double dt,
+                                                                       double total_dv, double Tmean,
+                                                                       double vol, double vol_old,
+                                                                       double *s, double &syy,
+                                                                       const double *edot, double &plstrain,
+                                                                       double &delta_plstrain,
+                                                                       double &dpressure)
 {
     constexpr int NSTR = Dim<ND>::NSTR;
     const des_params *P = st.P;
     double old_s = trace<ND>(s);
-    {
-        double div = trace<ND>(edot);
-#pragma unroll
-        for (int i = 0; i < ND; ++i) edot[i] += (edvoldt - div) / ND;
-    }
-#pragma unroll
-    for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dt;
     double de[NSTR];
 #pragma unroll
     for (int i = 0; i < NSTR; ++i) de[i] = edot[i] * dt;
@@ -871,7 +871,6 @@ template <int ND> __device__ __forceinline__ bool stress_update(const DevState &
     case DES_RH_VISCOUS: {
         double bulkm = harmonic_mean(st, P->bulk_modulus, e);
         double viscosity = mat_visc<ND>(st, e, Tmean, edot);
-        double total_dv = trace<ND>(es);
         viscous<ND>(bulkm, viscosity, total_dv, edot, s);
         break;
     }
@@ -933,6 +932,27 @@ template <int ND> __device__ __forceinline__ bool stress_update(const DevState &
     return ok;
 }
 
+// Body of update_stress for one element (rheology.cxx:520-641).  edot is
+// anti-locked in place; returns false for an unknown rheology.
+template <int ND> __device__ __forceinline__ bool stress_update(const DevState &st, int e, double dt,
+                                                               double edvoldt, double Tmean,
+                                                               double vol, double vol_old,
+                                                               double *s, double &syy, double *es,
+                                                               double *edot, double &plstrain,
+                                                               double &delta_plstrain, double &dpressure)
+{
+    constexpr int NSTR = Dim<ND>::NSTR;
+    {
+        double div = trace<ND>(edot);
+#pragma unroll
+        for (int i = 0; i < ND; ++i) edot[i] += (edvoldt - div) / ND;
+    }
+#pragma unroll
+    for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dt;
+    return rheology_after_strain<ND>(st, e, dt, trace<ND>(es), Tmean, vol, vol_old, s, syy, edot, plstrain,
+                                     delta_plstrain, dpressure);
+}
+
 // ===================================================== mixed precision
 //
 // The stress is carried as an FP64 mean p of its diagonal plus an FP32
@@ -987,6 +1007,8 @@ template <int ND> __device__ __forceinline__ float mat_visc_f(const DevState &s,
     float visc;
     if (s.visc32) {
         visc = s.visc32[e];
+    } else if (P->visc_min == P->visc_max) {
+        visc = (float)P->visc_min;
     } else {
         float ed = sqrtf(second_invariant2_f<ND>(edot));
         ed = fmaxf(ed, 1e-30f);

@@ -63,6 +63,23 @@ def test_mixed_long_run_10k_steps():
     assert e["vel"] < 1e-4 and e["stress"] < 1e-4, e
 
 
+@pytest.mark.parametrize("name,nsteps", [("evp3", 300), ("maxwell3", 300), ("evp2", 300)])
+def test_fp64_sensitivity_floor(name, nsteps):
+    """How far two FP64 runs drift apart when the initial stress differs by
+    1e-12 relative: the model's own noise floor for trajectory comparisons."""
+    from dynearthsol_amd.lib import DesModel
+    m = SC.build(name)
+    a = DesModel(m)
+    b = DesModel(m)
+    s0 = m.stress0 * (1 + 1e-12 * (2 * M.splitmix_uniform(5, m.nelem)[:, None] - 1))
+    b.upload(stress=s0)
+    a.run(nsteps)
+    b.run(nsteps)
+    e = _errors(b.download(), a.download())
+    print(f"FP64 floor {name} {nsteps} steps:", {k: f"{v:.2e}" for k, v in e.items()})
+    assert np.isfinite(e["vel"])
+
+
 def test_mixed_against_oracle_short():
     m = SC.build("evp3")
     o = OracleState(m)
