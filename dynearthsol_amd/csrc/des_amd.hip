@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -34,11 +35,48 @@ inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
 // ---------------------------------------------------------------- kernels
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumbering them so that XCD x runs one contiguous range of logical
+// blocks gives every XCD the same contiguous slab of the (renumbered) mesh in
+// every pass: what one pass writes is re-read from the same L2 by the next.
+__device__ __forceinline__ int xcd_block()
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// Element / node index of this thread.  For batched models the block's model
+// supplies the parameters and scalars (the kernel's by-value DevState is
+// rebound), and padding lanes past the model's last element return.
+#define BIND_ELEM(s, e)                                \
+    const int blk_ = xcd_block();                      \
+    const int e = blk_ * BLOCK + threadIdx.x;          \
+    if (s.eblk_model) {                                \
+        if (e >= s.eblk_lim[blk_]) return;             \
+        const int m_ = s.eblk_model[blk_];             \
+        s.P = s.Pb + m_;                               \
+        s.sc = s.scb + m_;                             \
+    } else if (e >= s.E) {                             \
+        return;                                        \
+    }
+
+#define BIND_NODE(s, n)                                \
+    const int blk_ = xcd_block();                      \
+    const int n = blk_ * BLOCK + threadIdx.x;          \
+    if (s.nblk_model) {                                \
+        if (n >= s.nblk_lim[blk_]) return;             \
+        const int m_ = s.nblk_model[blk_];             \
+        s.P = s.Pb + m_;                               \
+        s.sc = s.scb + m_;                             \
+    } else if (n >= s.N) {                             \
+        return;                                        \
+    }
+
 // compute_volume (geometry.cxx:86-108)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_volume(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -50,8 +88,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_volume(DevState s)
 // compute_shape_fn (geometry.cxx:368-455)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_shape_fn(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -90,8 +127,7 @@ template <int ND> __device__ __forceinline__ void mass_of(const DevState &s, int
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_mass_elem(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     int c[Dim<ND>::NPE];
     load_conn<ND>(s, e, c);
     double m, tm;
@@ -103,8 +139,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_mass_elem(DevState 
 // node part of compute_mass (geometry.cxx:353-359)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_mass_node(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
     double vn = 0, m = 0, tm = 0;
@@ -148,8 +183,7 @@ template <int ND> __device__ __forceinline__ void temperature_contrib(const DevS
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_elem(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -175,8 +209,7 @@ __device__ __forceinline__ double temperature_new(const DevState &s, int n, doub
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_node(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     double tdot = 0;
     for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) tdot += s.ework[s.csr_ent[p]];
     s.ntmp[n] = tdot;
@@ -185,8 +218,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_node(DevState 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_strain_rate(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -201,8 +233,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_strain_rate(DevStat
 // compute_dvoldt (geometry.cxx:111-155), into ntmp
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_dvoldt_node(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
     for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
@@ -219,8 +250,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_dvoldt_node(DevStat
 // compute_edvoldt (geometry.cxx:158-179)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_edvoldt(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     int c[Dim<ND>::NPE];
     load_conn<ND>(s, e, c);
     double dj = 0;
@@ -251,8 +281,7 @@ __device__ __forceinline__ void set_error(DevScalars *sc, int code)
 // update_stress (rheology.cxx:510-646)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_stress(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     int c[NPE];
@@ -278,8 +307,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_stress(DevSt
 // NMD_stress (geometry.cxx:182-236): node average of dp into ntmp ...
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_nmd_node(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
     for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
@@ -304,8 +332,7 @@ template <int ND> __device__ __forceinline__ void nmd_correct(const DevState &s,
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_nmd_elem(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     int c[Dim<ND>::NPE];
     load_conn<ND>(s, e, c);
     double st[ND];
@@ -363,8 +390,7 @@ template <int ND> __device__ __forceinline__ void store_force_contrib(const DevS
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_elem(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -515,8 +541,7 @@ template <int ND> __device__ __forceinline__ void node_force(const DevState &s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_node(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     double f[ND], v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = s.vel[d * s.N + n];
@@ -527,8 +552,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_node(DevState
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_velocity(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     const double dt = s.sc->dt, m = s.mass[n];
 #pragma unroll
     for (int d = 0; d < ND; ++d) s.vel[d * s.N + n] += dt * s.force[d * s.N + n] / m;
@@ -671,8 +695,7 @@ template <int ND> __device__ __forceinline__ bool vbc_node(const DevState &s, un
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_apply_vbcs(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     unsigned flag = s.bcflag[n];
     if (!(flag & BOUND_ANY)) return;
     double v[ND];
@@ -685,8 +708,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_apply_vbcs(DevState
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_coordinate(DevState s)
 {
-    int n = blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     const double dt = s.sc->dt;
 #pragma unroll
     for (int d = 0; d < ND; ++d) s.coord[d * s.N + n] += s.vel[d * s.N + n] * dt;
@@ -695,8 +717,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_coordinate(D
 // rotate_stress (fields.cxx:411-483) with stored shape functions
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_rotate(DevState s, int use_dt_prev)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -838,6 +859,72 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial(Dev
     }
 }
 
+// per-element fields of an output frame (output.cxx:147-166): MatProps::rho,
+// elem_quality and MatProps::visc of the last step's strain rate
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_elem_diagnostics(DevState s, double *density,
+                                                                            double *quality, double *viscosity)
+{
+    BIND_ELEM(s, e);
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    const double Tm = elem_mean<ND>(s.T, c);
+    density[e] = mat_rho(s, e, Tm);
+    double sr[NSTR];
+    load_tensor<ND>(s.strain_rate, s.E, e, sr);
+    viscosity[e] = mat_visc<ND>(s, e, Tm, sr);
+    double x[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    const double vol = s.volume[e];
+    if constexpr (ND == 3) {
+        double normalization_factor = 216 * sqrt(3.0);
+        double area_sum = (triangle_area<3>(x[0], x[1], x[2]) + triangle_area<3>(x[0], x[1], x[3]) +
+                           triangle_area<3>(x[2], x[3], x[0]) + triangle_area<3>(x[2], x[3], x[1]));
+        quality[e] = normalization_factor * vol * vol / (area_sum * area_sum * area_sum);
+    } else {
+        double normalization_factor = 4 * sqrt(3.0);
+        double dist2_sum = dist2<2>(x[0], x[1]) + dist2<2>(x[1], x[2]) + dist2<2>(x[0], x[2]);
+        quality[e] = normalization_factor * vol / dist2_sum;
+    }
+}
+
+// ---- bad_mesh_quality's first two checks (remeshing.cxx:1893-1918): the
+// first element smaller than smallest_vol and the first bottom node farther
+// than dist from the initial bottom depth, as block minima of their indices
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_badmesh_partial(DevState s, double smallest_vol,
+                                                                           int check_bottom, double bottom,
+                                                                           double dist)
+{
+    int first_e = 0x7fffffff, first_n = 0x7fffffff;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK)
+        if (s.volume[e] < smallest_vol) {
+            first_e = min(first_e, e);
+            break;
+        }
+    if (check_bottom) {
+        for (int n = blockIdx.x * BLOCK + threadIdx.x; n < s.N; n += gridDim.x * BLOCK)
+            if ((s.bcflag[n] & BOUNDZ0) && fabs(s.coord[(ND - 1) * s.N + n] - bottom) > dist) {
+                first_n = min(first_n, n);
+                break;
+            }
+    }
+    __shared__ int se[BLOCK], sn[BLOCK];
+    se[threadIdx.x] = first_e;
+    sn[threadIdx.x] = first_n;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            se[threadIdx.x] = min(se[threadIdx.x], se[threadIdx.x + w]);
+            sn[threadIdx.x] = min(sn[threadIdx.x], sn[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s.redi[2 * blockIdx.x] = se[0];
+        s.redi[2 * blockIdx.x + 1] = sn[0];
+    }
+}
+
 // ============================================================ fused step
 //
 // Pass A (elements): volume, shape functions, thermal contributions,
@@ -855,17 +942,6 @@ struct FusedFlags {
     int store_aux;         // last step before a sync: keep strain_rate, edvoldt, force
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Renumbering them so that XCD x runs one contiguous range of logical
-// blocks gives every XCD the same contiguous slab of the (renumbered) mesh in
-// every pass: what one pass writes is re-read from the same L2 by the next.
-__device__ __forceinline__ int xcd_block()
-{
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
-
 // Packed per-element record written by pass A and gathered by pass B: the
 // thermal contributions to the element's nodes, tr(edot) V, the element's
 // mass and thermal mass and its volume, one 64-byte line per element.
@@ -874,8 +950,7 @@ constexpr int REC_DJ = 4, REC_M = 5, REC_TM = 6, REC_V = 7;
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
 {
-    int e = xcd_block() * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const des_params *P = s.P;
     int c[NPE];
@@ -911,8 +986,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
 {
-    int n = xcd_block() * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
     double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
@@ -1005,16 +1079,14 @@ template <int ND> __device__ __forceinline__ void pass_c_body(const DevState &s,
 
 template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
-    int e = xcd_block() * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     pass_c_body<ND>(s, fl, e);
 }
 
 // pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s)
 {
-    int n = xcd_block() * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
     for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) sum += s.escal[s.csr_ent[p] / NPE];
@@ -1023,8 +1095,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s)
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
 {
-    int e = xcd_block() * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     int c[NPE];
@@ -1044,8 +1115,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, FusedFlags fl)
 {
-    int n = xcd_block() * BLOCK + threadIdx.x;
-    if (n >= s.N) return;
+    BIND_NODE(s, n);
     const double dt = s.sc->dt;
     double f[ND], v[ND];
 #pragma unroll
@@ -1072,8 +1142,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_to_mixed(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double st[NSTR], p;
@@ -1094,8 +1163,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_to_mixed(DevState s
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState s)
 {
-    int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     float sd[NSTR];
@@ -1112,8 +1180,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevState s, FusedFlags fl)
 {
-    int e = xcd_block() * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     int c[NPE];
@@ -1182,8 +1249,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
 {
-    int e = xcd_block() * BLOCK + threadIdx.x;
-    if (e >= s.E) return;
+    BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     int c[NPE];
@@ -1249,6 +1315,8 @@ struct des_ctx {
     double *elem0 = nullptr;       // bulkm(0), rho(0)
     int steps = 0;                 // host mirror of var.steps
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
+    std::vector<int> h_conn;       // host copies for output frames
+    std::vector<unsigned> h_bcflag;
     int passc_variant = -1;        // waves-per-EU variant of pass C chosen by the autotuner
     double passc_ms[4] = {0, 0, 0, 0};
 };
@@ -1583,6 +1651,8 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
     ALLOC(s.redi, RED_BLOCKS);
     ALLOC(ctx->elem0, 2);
 #undef ALLOC
+    ctx->h_conn.assign(m->connectivity, m->connectivity + (size_t)E * npe);
+    ctx->h_bcflag.assign(m->bcflag, m->bcflag + N);
     // initial coordinates, also coord0 (dynearthsol.cxx:82-84)
     if ((rc = upload_soa(ctx, s.coord, m->coord, N, nd))) return rc;
     if ((rc = upload_soa(ctx, s.coord0, m->coord, N, nd))) return rc;
@@ -1666,6 +1736,112 @@ int des_download_fields(des_ctx *ctx, des_fields *f)
     double *shp_dst[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
     for (int d = 0; d < nd; ++d)
         if ((rc = download_soa(ctx, shp_dst[d], s.shp + (size_t)d * npe * E, E, npe))) return rc;
+    return DES_OK;
+}
+
+// One output frame in the reference's binary format (binaryio.cxx:16-36: a
+// 4096-byte ASCII header "# DynEarthSol ndims=N revision=3" followed by
+// "name<TAB>offset" lines, then the raw arrays) with the field set of
+// Output::write (output.cxx:93-179, instantaneous fields), and one line of
+// the .info table (output.cxx:43-76).
+int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_time)
+{
+    if (!ctx || !modelname || frame < 0) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    const long N = ctx->N, E = ctx->E;
+    const int nd = ctx->nd, npe = ctx->npe, nstr = ctx->nstr;
+    std::vector<double> coord(N * nd), vel(N * nd), force(N * nd), coord0(N * nd), T(N);
+    std::vector<double> pl(E), dpl(E), sr(E * nstr), strain(E * nstr), stress(E * nstr);
+    std::vector<double> density(E), quality(E), viscosity(E), material(E, 0.0);
+    des_fields f{};
+    f.coord = coord.data(); f.vel = vel.data(); f.force = force.data(); f.coord0 = coord0.data();
+    f.temperature = T.data(); f.plstrain = pl.data(); f.delta_plstrain = dpl.data();
+    f.strain_rate = sr.data(); f.strain = strain.data(); f.stress = stress.data();
+    if ((rc = des_download_fields(ctx, &f))) return rc;
+    double *d_diag;
+    HIPCHK(hipMallocAsync((void **)&d_diag, sizeof(double) * 3 * E, ctx->stream));
+    LAUNCH_ND(k_elem_diagnostics, nblocks(E), ctx->s, d_diag, d_diag + E, d_diag + 2 * E);
+    HIPCHK(hipMemcpyAsync(density.data(), d_diag, sizeof(double) * E, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(quality.data(), d_diag + E, sizeof(double) * E, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(viscosity.data(), d_diag + 2 * E, sizeof(double) * E, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipFreeAsync(d_diag, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->s.elemmarkers) {       // most abundant material (output.cxx:168-173)
+        std::vector<int> em((size_t)E * ctx->P.nmat);
+        HIPCHK(hipMemcpy(em.data(), ctx->s.elemmarkers, sizeof(int) * em.size(), hipMemcpyDeviceToHost));
+        for (long e = 0; e < E; ++e) {
+            const int *a = em.data() + e * ctx->P.nmat;
+            material[e] = (double)(std::max_element(a, a + ctx->P.nmat) - a);
+        }
+    }
+    DevScalars sc{};
+    if ((rc = read_scalars(ctx, sc))) return rc;
+
+    char fname[1024];
+    std::snprintf(fname, sizeof fname, "%s.save.%06d", modelname, frame);
+    std::FILE *fp = std::fopen(fname, "wb");
+    if (!fp) return fail(ctx, DES_ERR_IO, std::string("cannot open ") + fname);
+    const size_t headerlen = 4096;
+    std::string header = "# DynEarthSol ndims=" + std::to_string(nd) + " revision=3\n";
+    long eof = (long)headerlen;
+    std::fseek(fp, eof, SEEK_SET);
+    bool ok = true;
+    auto put = [&](const char *name, const void *data, size_t bytes) {
+        char line[256];
+        std::snprintf(line, sizeof line, "%s\t%ld\n", name, eof);
+        header += line;
+        if (std::fwrite(data, 1, bytes, fp) != bytes) ok = false;
+        eof += (long)bytes;
+    };
+    put("coordinate", coord.data(), sizeof(double) * coord.size());
+    put("connectivity", ctx->h_conn.data(), sizeof(int) * ctx->h_conn.size());
+    put("velocity", vel.data(), sizeof(double) * vel.size());
+    put("temperature", T.data(), sizeof(double) * T.size());
+    put("plastic strain", pl.data(), sizeof(double) * pl.size());
+    put("plastic strain-rate", dpl.data(), sizeof(double) * dpl.size());
+    put("strain-rate", sr.data(), sizeof(double) * sr.size());
+    put("strain", strain.data(), sizeof(double) * strain.size());
+    put("stress", stress.data(), sizeof(double) * stress.size());
+    put("density", density.data(), sizeof(double) * E);
+    put("mesh quality", quality.data(), sizeof(double) * E);
+    put("viscosity", viscosity.data(), sizeof(double) * E);
+    put("material", material.data(), sizeof(double) * E);
+    put("force", force.data(), sizeof(double) * force.size());
+    put("coord0", coord0.data(), sizeof(double) * coord0.size());
+    put("bcflag", ctx->h_bcflag.data(), sizeof(unsigned) * ctx->h_bcflag.size());
+    if (header.size() >= headerlen) ok = false;
+    header.resize(headerlen, '\0');
+    std::fseek(fp, 0, SEEK_SET);
+    if (std::fwrite(header.data(), 1, headerlen, fp) != headerlen) ok = false;
+    std::fclose(fp);
+    if (!ok) return fail(ctx, DES_ERR_IO, std::string("failed writing ") + fname);
+
+    std::string info = std::string(modelname) + ".info";
+    std::FILE *fi = std::fopen(info.c_str(), frame == 0 ? "w" : "a");
+    if (!fi) return fail(ctx, DES_ERR_IO, "cannot open " + info);
+    std::fprintf(fi, "%6d\t%10d\t%12.6e\t%12.4e\t%12.6e\t%8d\t%8d\t%8d\n", frame, sc.steps, sc.time, sc.dt,
+                 run_time, ctx->N, ctx->E, ctx->nbf);
+    std::fclose(fi);
+    for (double x : coord)      // output.cxx:194-208
+        if (!std::isfinite(x)) return fail(ctx, DES_ERR_RUNTIME, "coordinate becomes NaN or Infinity");
+    return DES_OK;
+}
+
+int des_upload_elemmarkers(des_ctx *ctx, const int *counts)
+{
+    if (!ctx || !counts) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    const size_t n = (size_t)ctx->E * ctx->P.nmat;
+    if (!ctx->s.elemmarkers) {
+        int *em;
+        if ((rc = dalloc_n(ctx, &em, n))) return rc;
+        ctx->s.elemmarkers = em;
+    }
+    HIPCHK(hipMemcpyAsync(const_cast<int *>(ctx->s.elemmarkers), counts, sizeof(int) * n,
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
 }
 
@@ -1772,6 +1948,49 @@ int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem)
         }
     if (q) *q = best;
     if (worst_elem) *worst_elem = bi;
+    return DES_OK;
+}
+
+int des_bad_mesh_quality(des_ctx *ctx, int remeshing_option, double max_boundary_distortion,
+                         int *reason, int *index)
+{
+    if (!ctx || !reason || !index) return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    const des_params &P = ctx->P;
+    const double sizefactor = ctx->nd == 3 ? 0.118 : 0.433;   // remeshing.cxx:76-82
+    const double smallest_vol = P.smallest_size * sizefactor * std::pow(P.resolution, ctx->nd);
+    const int check_bottom = remeshing_option == 1 || remeshing_option == 2 || remeshing_option == 11;
+    LAUNCH_ND(k_badmesh_partial, RED_BLOCKS, ctx->s, smallest_vol, check_bottom, -P.zlength,
+              max_boundary_distortion * P.resolution);
+    std::vector<int> h(2 * RED_BLOCKS);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->s.redi, sizeof(int) * 2 * RED_BLOCKS, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int fe = 0x7fffffff, fn = 0x7fffffff;
+    for (int b = 0; b < RED_BLOCKS; ++b) {
+        fe = std::min(fe, h[2 * b]);
+        fn = std::min(fn, h[2 * b + 1]);
+    }
+    *reason = 0;
+    *index = 0;
+    if (fe != 0x7fffffff) {
+        *reason = 3;
+        *index = fe;
+        return DES_OK;
+    }
+    if (fn != 0x7fffffff) {
+        *reason = 2;
+        *index = fn;
+        return DES_OK;
+    }
+    double q;
+    int worst;
+    if ((rc = des_worst_quality(ctx, &q, &worst))) return rc;
+    if (ctx->nd == 3) q = std::pow(q, 1.0 / 3);      // remeshing.cxx:1924-1927
+    if (q < P.min_quality) {
+        *reason = 1;
+        *index = worst;
+    }
     return DES_OK;
 }
 

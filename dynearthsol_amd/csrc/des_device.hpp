@@ -73,6 +73,12 @@ struct DevState {
     float *strain32;                              // NSTR x E
     float *pl32, *dpl32;                          // E
     const float *visc32, *coh32;                  // E or null
+    // batched models (null for a single model): elements and nodes of model m
+    // fill whole blocks; block b of an element (node) kernel belongs to model
+    // eblk_model[b] (nblk_model[b]) and holds real elements below eblk_lim[b]
+    const int *eblk_model, *eblk_lim, *nblk_model, *nblk_lim;
+    const des_params *Pb;                         // per-model parameters
+    DevScalars *scb;                              // per-model scalars
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min

@@ -22,7 +22,7 @@ EXPORTS = (
     "des_nmd_stress", "des_update_force", "des_update_velocity",
     "des_apply_vbcs", "des_update_mesh", "des_rotate_stress",
     "des_compute_dt", "des_worst_quality", "des_step", "des_run", "des_run_timed",
-    "des_sync",
+    "des_sync", "des_bad_mesh_quality", "des_upload_elemmarkers", "des_write_frame",
 )
 
 PER_FUNCTION = (
@@ -68,6 +68,9 @@ def load():
     lib.des_worst_quality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.des_run.argtypes = [vp, C.c_int, C.c_int]
     lib.des_run_timed.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.des_bad_mesh_quality.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.des_upload_elemmarkers.argtypes = [vp, C.POINTER(C.c_int)]
+    lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     _lib = lib
     return lib
 
@@ -162,6 +165,22 @@ class DesModel:
         dt, t, st = C.c_double(), C.c_double(), C.c_int()
         self._check(self.lib.des_get_scalars(self.ctx, C.byref(dt), C.byref(t), C.byref(st)))
         return dt.value, t.value, st.value
+
+    def bad_mesh_quality(self, remeshing_option=0, max_boundary_distortion=0.25):
+        """(reason, index) as bad_mesh_quality (remeshing.cxx:1882-1934)."""
+        r, i = C.c_int(), C.c_int()
+        self._check(self.lib.des_bad_mesh_quality(self.ctx, remeshing_option, max_boundary_distortion,
+                                                  C.byref(r), C.byref(i)))
+        return r.value, i.value
+
+    def write_frame(self, modelname, frame, run_time=0.0):
+        """<modelname>.save.<frame> and a line of <modelname>.info (output.cxx:79-208)."""
+        self._check(self.lib.des_write_frame(self.ctx, str(modelname).encode(), frame, run_time))
+
+    def upload_elemmarkers(self, counts):
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        assert counts.shape == (self.E, self.params.nmat)
+        self._check(self.lib.des_upload_elemmarkers(self.ctx, abi.iptr(counts)))
 
     def worst_quality(self):
         q, w = C.c_double(), C.c_int()

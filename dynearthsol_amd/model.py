@@ -657,6 +657,49 @@ class Model:
         return m
 
 
+def config_ensemble(n2d=128, n3d=128, seed=6, size_scale=1.0):
+    """The batched ensemble: n2d 2D models with log-uniform 1k-50k triangles and
+    n3d 3D models with log-uniform 10k-200k tetrahedra, each with its own
+    extension rate U[0.5, 5] cm/yr, friction angle U[20, 35] deg and weak-seed
+    x-centre U[0.3, 0.7], drawn from a counter-based generator keyed by seed."""
+    n = n2d + n3d
+    u = [splitmix_uniform(seed, n, k) for k in range(4)]
+    models = []
+    for i in range(n):
+        nd = 2 if i < n2d else 3
+        lo, hi = (1e3, 5e4) if nd == 2 else (1e4, 2e5)
+        ne = int(round(size_scale * math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * u[0][i])))
+        rate = (0.5 + 4.5 * u[1][i]) * 1e-2 / YEAR2SEC
+        phi = 20.0 + 15.0 * u[2][i]
+        xc = 0.3 + 0.4 * u[3][i]
+        cfg = {
+            "mat.rheology_type": "elasto-visco-plastic",
+            "mat.rho0": [2800.0], "mat.bulk_modulus": [50e9], "mat.shear_modulus": [30e9],
+            "mat.cohesion0": [44e6], "mat.cohesion1": [4e6],
+            "mat.friction_angle0": [phi], "mat.friction_angle1": [5.0],
+            "mat.pls0": [0.0], "mat.pls1": [0.1],
+            "mat.min_viscosity": 1e24, "mat.max_viscosity": 1e24,
+            "mat.heat_capacity": [1000.0], "mat.therm_cond": [2.8],
+            "bc.surface_temperature": 273.0, "bc.mantle_temperature": 1573.0,
+            "bc.has_fixed_bottom_temperature": False,
+            "bc.vbc_x0": 1, "bc.vbc_val_x0": -rate, "bc.vbc_x1": 1, "bc.vbc_val_x1": rate,
+            "bc.vbc_z0": 1, "bc.vbc_val_z0": 0.0, "bc.vbc_z1": 0,
+            "bc.has_winkler_foundation": False, "bc.has_water_loading": False,
+            "mesh.xlength": 100e3, "mesh.zlength": 10e3, "mesh.resolution": 1e3,
+            "ic.weakzone_xcenter": xc,
+        }
+        if nd == 2:
+            cfg.update({"ic.weakzone_option": 2, "ic.weakzone_xsemi_axis": 2.5e3,
+                        "ic.weakzone_zsemi_axis": 0.5e3, "ic.weakzone_zcenter": 0.5})
+        else:
+            cfg.update({"mesh.ylength": 100e3, "bc.vbc_y0": 1, "bc.vbc_val_y0": 0.0,
+                        "bc.vbc_y1": 1, "bc.vbc_val_y1": 0.0,
+                        "ic.weakzone_option": 1, "ic.weakzone_halfwidth": 2.5,
+                        "ic.weakzone_depth_min": 0.45, "ic.weakzone_depth_max": 0.55})
+        models.append(Model(cfg, nd, nelem_target=ne, temperature="linear", mesh_seed=i))
+    return models
+
+
 def config_benchmark(which, nelem_target=None):
     """The workloads the benchmark and the tests are quoted on.
 

@@ -156,6 +156,16 @@ int des_upload_fields(des_ctx *ctx, const des_fields *f);
    arrays equal the reference's Variables after the same number of steps. */
 int des_download_fields(des_ctx *ctx, des_fields *f);
 
+/* Output::write (output.cxx:79-208) of the instantaneous fields: writes
+   "<modelname>.save.<frame %06d>" in the reference's binary format
+   (binaryio.cxx:16-36) and appends the frame's line to "<modelname>.info"
+   (created at frame 0).  Returns 11 when a coordinate is NaN or infinite. */
+int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_time);
+
+/* Marker counts per element and material (var.elemmarkers, nelem x nmat),
+   e.g. after the host's phase changes (phasechanges.cxx:232-258). */
+int des_upload_elemmarkers(des_ctx *ctx, const int *counts);
+
 /* Scalar state: var.dt, var.time, var.steps, var.compensation_pressure. */
 int des_set_scalars(des_ctx *ctx, double dt, double time, int steps,
                     double compensation_pressure);
@@ -182,6 +192,13 @@ int des_update_mesh(des_ctx *ctx);          /* dynearthsol.cxx:221-232 */
 int des_rotate_stress(des_ctx *ctx);        /* fields.cxx:411-483 */
 int des_compute_dt(des_ctx *ctx, double *dt_out);   /* geometry.cxx:239-310; sets var.dt */
 int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem); /* geometry.cxx:499-512 */
+/* bad_mesh_quality (remeshing.cxx:1882-1934): *reason = 3 (an element smaller
+   than mesh.smallest_size of the resolution), 2 (a bottom node drifted more than
+   max_boundary_distortion * resolution, remeshing options 1, 2, 11), 1 (worst
+   quality, cube-rooted in 3D, below mesh.min_quality) or 0; *index names the
+   element or node. */
+int des_bad_mesh_quality(des_ctx *ctx, int remeshing_option, double max_boundary_distortion,
+                         int *reason, int *index);
 
 /* ---- step drivers --------------------------------------------------------- */
 

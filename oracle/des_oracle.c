@@ -552,6 +552,47 @@ double FN(worst_elem_quality)(ostate *S, int *worst_elem) /* geometry.cxx:499-51
     return q;
 }
 
+/* bad_mesh_quality (remeshing.cxx:1882-1934): returns 3, 2, 1 or 0 */
+int FN(bad_mesh_quality)(ostate *S, const des_params *P, int remeshing_option,
+                         double max_boundary_distortion, int *index)
+{
+#if NDIMS == 3
+    const double sizefactor = 0.118;
+#else
+    const double sizefactor = 0.433;
+#endif
+    const double smallest_vol = P->smallest_size * sizefactor * pow(P->resolution, NDIMS);
+    for (int e = 0; e < S->nelem; e++) {
+        if (S->volume[e] < smallest_vol) {
+            *index = e;
+            return 3;
+        }
+    }
+    if (remeshing_option == 1 || remeshing_option == 2 || remeshing_option == 11) {
+        double bottom = -P->zlength;
+        const double dist = max_boundary_distortion * P->resolution;
+        for (int i = 0; i < S->nnode; ++i) {
+            if (S->bcflag[i] & BOUNDZ0) {
+                double z = S->coord[NDIMS * i + NDIMS - 1];
+                if (fabs(z - bottom) > dist) {
+                    *index = i;
+                    return 2;
+                }
+            }
+        }
+    }
+    int worst_elem;
+    double q = FN(worst_elem_quality)(S, &worst_elem);
+#if NDIMS == 3
+    q = pow(q, 1.0 / 3);
+#endif
+    if (q < P->min_quality) {
+        *index = worst_elem;
+        return 1;
+    }
+    return 0;
+}
+
 /* -------------------------------------------------------------- fields.cxx */
 
 void FN(update_temperature)(ostate *S, const des_params *P)   /* fields.cxx:96-163 */

@@ -83,6 +83,8 @@ def oracle_lib():
         getattr(lib, pf + "mat_rho").restype = C.c_double
         getattr(lib, pf + "mat_plastic_props").argtypes = [C.c_void_p, C.c_void_p, C.c_int,
                                                             C.c_double, _d]
+        getattr(lib, pf + "bad_mesh_quality").argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, _i]
+        getattr(lib, pf + "bad_mesh_quality").restype = C.c_int
         getattr(lib, pf + "sizeof_state").restype = C.c_int
         assert getattr(lib, pf + "sizeof_state")() == C.sizeof(OState)
     lib.o3_dsyevh3.argtypes = [_d, _d, _d]
@@ -222,6 +224,12 @@ class OracleState:
         rc = getattr(self.lib, self.pf + "run")(C.byref(self.S), C.byref(self.params), n)
         self._sync_swap()
         return rc
+
+    def bad_mesh_quality(self, remeshing_option=0, max_boundary_distortion=0.25):
+        i = C.c_int(0)
+        r = getattr(self.lib, self.pf + "bad_mesh_quality")(
+            C.byref(self.S), C.byref(self.params), remeshing_option, max_boundary_distortion, C.byref(i))
+        return r, i.value
 
     def worst_quality(self):
         w = C.c_int()
