@@ -40,6 +40,9 @@ PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
 WORKLOADS = {
     "3d-fp64": ("3d-fp64 extension box 100x100x10 km, EVP Mohr-Coulomb", 1_000_000),
     "3d-mixed": ("3d-mixed box 200x200x20 km, per-element viscosity and cohesion, EVP", 8_000_000),
+    "ensemble": ("ensemble of 256 box models (128 2D with 1k-50k triangles, 128 3D with 10k-200k "
+                 "tetrahedra, log-uniform sizes, per-model extension rate, friction angle and "
+                 "weak-seed position, seed 6), mixed precision", None),
 }
 
 
@@ -79,6 +82,51 @@ def rel(a, b):
     return float(np.max(np.abs(a - b))) / s if s else float(np.max(np.abs(a - b)))
 
 
+def bench_ensemble(args):
+    """All models in one batch (des_batch_run) against the same models run one
+    at a time on their own contexts (des_run), both in mixed precision."""
+    from dynearthsol_amd import abi, model as M
+    from dynearthsol_amd.lib import DesBatch, DesModel
+    t0 = time.perf_counter()
+    models = M.config_ensemble(size_scale=args.ensemble_scale)
+    n2 = sum(m.ndims == 2 for m in models)
+    E = sum(m.nelem for m in models)
+    print(f"# ensemble: {len(models)} models, {E} elements, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    bt = DesBatch(models)
+    bt.run(args.warmup, abi.DES_MIXED)
+    bt.sync()
+    t0 = time.perf_counter()
+    bt.run(args.steps, abi.DES_MIXED)
+    bt.sync()
+    secs = time.perf_counter() - t0
+    value = E * args.steps / secs
+    print(f"# batch: {1e3 * secs / args.steps:.4f} ms per step of all models", file=sys.stderr, flush=True)
+    bt.close()
+    # one at a time: every model, seq_steps timed steps each after a short warmup
+    seq_secs, seq_work = 0.0, 0
+    for m in models:
+        g = DesModel(m)
+        g.run(20, abi.DES_MIXED)
+        dt = timed(g, args.seq_steps, abi.DES_MIXED)
+        seq_secs += dt
+        seq_work += m.nelem * args.seq_steps
+        g.close()
+    seq_value = seq_work / seq_secs
+    print(f"# one at a time: {seq_value:.4e} elem-steps/s", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC, "value": value, "unit": "elem-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 deviator/rheology + f64 pressure/geometry/force", "data": "synthetic",
+        "config": {"workload": WORKLOADS["ensemble"][0], "models": len(models), "models_2d": n2,
+                   "nelem_total": E, "precision": "mixed", "schedule": "batched fused six-pass"},
+        "one_at_a_time": {"value": seq_value, "steps_per_model": args.seq_steps,
+                          "batch_speedup": value / seq_value},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +137,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--ensemble-scale", type=float, default=1.0)
+    ap.add_argument("--seq-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
               "not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
 
     from dynearthsol_amd import abi, model as M
     from dynearthsol_amd.lib import DesModel

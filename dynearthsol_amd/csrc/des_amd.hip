@@ -67,11 +67,20 @@ __device__ __forceinline__ int xcd_block()
     if (s.nblk_model) {                                \
         if (n >= s.nblk_lim[blk_]) return;             \
         const int m_ = s.nblk_model[blk_];             \
-        s.P = s.Pb + m_;                               \
-        s.sc = s.scb + m_;                             \
+        bind_model<ND>(s, m_);                         \
     } else if (n >= s.N) {                             \
         return;                                        \
     }
+
+// parameters, scalars and boundary geometry of batched model m
+template <int ND> __device__ __forceinline__ void bind_model(DevState &s, int m)
+{
+    s.P = s.Pb + m;
+    s.sc = s.scb + m;
+    s.bnormals = s.bnb + m * (DES_NBDRY * ND);
+    s.edge_vectors = s.evb + m * (DES_NBDRY * DES_NBDRY * ND);
+    s.edge_valid = s.evvb + m * (DES_NBDRY * DES_NBDRY);
+}
 
 // compute_volume (geometry.cxx:86-108)
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_volume(DevState s)
@@ -443,6 +452,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevS
     int t = blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nf) return;
     int k = facets[t];
+    if (s.fmodel) bind_model<ND>(s, s.fmodel[t]);
     const des_params *P = s.P;
     int i = s.bf_bdry[k], e = s.bf_elem[k], f = s.bf_face[k];
     constexpr int NPE = Dim<ND>::NPE;
@@ -718,6 +728,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_coordinate(D
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_rotate(DevState s, int use_dt_prev)
 {
     BIND_ELEM(s, e);
+    // batched models close their steps in one launch: only the elastic ones
+    // rotate (dynearthsol.cxx:346-347)
+    if (s.eblk_model && !(s.P->rheol_type & DES_RH_ELASTIC)) return;
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     int c[NPE];
     load_conn<ND>(s, e, c);
@@ -735,50 +748,33 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_rotate(DevState s, 
 }
 
 // ---- compute_dt (geometry.cxx:239-310): block minima, then one block
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial(DevState s)
+template <int ND> __device__ __forceinline__ void dt_terms(const DevState &s, int e, double &dtm, double &dtd,
+                                                         double &minl)
 {
     constexpr int NPE = Dim<ND>::NPE;
     const des_params *P = s.P;
-    double dtm = DBL_MAX, dtd = DBL_MAX, minl = DBL_MAX;
-    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) {
-        int c[NPE];
-        load_conn<ND>(s, e, c);
-        double x[NPE][ND];
-        gather_vec<ND>(s.coord, s.N, c, x);
-        double vol = elem_volume<ND>(x);
-        double minh;
-        if constexpr (ND == 3) {
-            double maxa = dmax(dmax(triangle_area<3>(x[0], x[1], x[2]), triangle_area<3>(x[0], x[1], x[3])),
-                               dmax(triangle_area<3>(x[2], x[3], x[0]), triangle_area<3>(x[2], x[3], x[1])));
-            minh = 3 * vol / maxa;
-        } else {
-            double maxl = sqrt(dmax(dmax(dist2<2>(x[0], x[1]), dist2<2>(x[1], x[2])), dist2<2>(x[0], x[2])));
-            minh = 2 * vol / maxl;
-        }
-        dtm = dmin(dtm, 0.5 * P->visc_min / (1e-40 + harmonic_mean(s, P->shear_modulus, e)));
-        if (P->has_thermal_diffusion) dtd = dmin(dtd, 0.5 * minh * minh / P->therm_diff_max);
-        minl = dmin(minl, minh);
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    double vol = elem_volume<ND>(x);
+    double minh;
+    if constexpr (ND == 3) {
+        double maxa = dmax(dmax(triangle_area<3>(x[0], x[1], x[2]), triangle_area<3>(x[0], x[1], x[3])),
+                           dmax(triangle_area<3>(x[2], x[3], x[0]), triangle_area<3>(x[2], x[3], x[1])));
+        minh = 3 * vol / maxa;
+    } else {
+        double maxl = sqrt(dmax(dmax(dist2<2>(x[0], x[1]), dist2<2>(x[1], x[2])), dist2<2>(x[0], x[2])));
+        minh = 2 * vol / maxl;
     }
-    __shared__ double sm[3][BLOCK];
-    sm[0][threadIdx.x] = dtm;
-    sm[1][threadIdx.x] = dtd;
-    sm[2][threadIdx.x] = minl;
-    __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int j = 0; j < 3; ++j) sm[j][threadIdx.x] = dmin(sm[j][threadIdx.x], sm[j][threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x < 3) s.red[blockIdx.x * 3 + threadIdx.x] = sm[threadIdx.x][0];
+    dtm = dmin(dtm, 0.5 * P->visc_min / (1e-40 + harmonic_mean(s, P->shear_modulus, e)));
+    if (P->has_thermal_diffusion) dtd = dmin(dtd, 0.5 * minh * minh / P->therm_diff_max);
+    minl = dmin(minl, minh);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb, double bulkm0, double rho0)
+// minimum of v[0..2] over the block, left in sm[j][0]
+__device__ __forceinline__ void block_min3(double (*sm)[BLOCK], const double v[3])
 {
-    const des_params *P = s.P;
-    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
-    for (int b = threadIdx.x; b < nb; b += BLOCK)
-        for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
-    __shared__ double sm[3][BLOCK];
     for (int j = 0; j < 3; ++j) sm[j][threadIdx.x] = v[j];
     __syncthreads();
     for (int w = BLOCK / 2; w > 0; w >>= 1) {
@@ -786,20 +782,76 @@ __global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb, double b
             for (int j = 0; j < 3; ++j) sm[j][threadIdx.x] = dmin(sm[j][threadIdx.x], sm[j][threadIdx.x + w]);
         __syncthreads();
     }
+}
+
+__device__ __forceinline__ void store_dt(const des_params *P, DevScalars *sc, double dt_maxwell,
+                                         double dt_diffusion, double minl, double bulkm0, double rho0)
+{
+    double dt;
+    if (P->fixed_dt != 0) {
+        dt = P->fixed_dt;
+    } else {
+        double dt_advection = 0.5 * minl / P->max_vbc_val;
+        double dt_elastic = P->is_quasi_static ?
+            0.5 * minl / (P->max_vbc_val * P->inertial_scaling) :
+            0.5 * minl / sqrt(bulkm0 / rho0);
+        dt = dmin(dmin(dt_elastic, dt_maxwell), dmin(dt_advection, dt_diffusion)) * P->dt_fraction;
+        if (!(dt > 0)) set_error(sc, DES_ERR_RUNTIME);
+    }
+    sc->dt = dt;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial(DevState s)
+{
+    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) dt_terms<ND>(s, e, v[0], v[1], v[2]);
+    __shared__ double sm[3][BLOCK];
+    block_min3(sm, v);
+    if (threadIdx.x < 3) s.red[blockIdx.x * 3 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb, double bulkm0, double rho0)
+{
+    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int b = threadIdx.x; b < nb; b += BLOCK)
+        for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
+    __shared__ double sm[3][BLOCK];
+    block_min3(sm, v);
+    if (threadIdx.x == 0) store_dt(s.P, s.sc, sm[0][0], sm[1][0], sm[2][0], bulkm0, rho0);
+}
+
+// compute_dt of batched models: element block b holds elements of one model
+// only, so its minima go to slot b; one block per model then reduces that
+// model's slots and evaluates bulkm and rho of the model's first element.
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial_b(DevState s)
+{
+    const int blk = xcd_block();
+    const int e = blk * BLOCK + threadIdx.x;
+    bind_model<ND>(s, s.eblk_model[blk]);
+    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    if (e < s.eblk_lim[blk]) dt_terms<ND>(s, e, v[0], v[1], v[2]);
+    __shared__ double sm[3][BLOCK];
+    block_min3(sm, v);
+    if (threadIdx.x < 3) s.red[blk * 3 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final_b(DevState s)
+{
+    const int m = blockIdx.x;
+    bind_model<ND>(s, m);
+    const int b0 = s.mblk_off[m], b1 = s.mblk_off[m + 1];
+    double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int b = b0 + threadIdx.x; b < b1; b += BLOCK)
+        for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
+    __shared__ double sm[3][BLOCK];
+    block_min3(sm, v);
     if (threadIdx.x == 0) {
-        double dt;
-        if (P->fixed_dt != 0) {
-            dt = P->fixed_dt;
-        } else {
-            double dt_maxwell = sm[0][0], dt_diffusion = sm[1][0], minl = sm[2][0];
-            double dt_advection = 0.5 * minl / P->max_vbc_val;
-            double dt_elastic = P->is_quasi_static ?
-                0.5 * minl / (P->max_vbc_val * P->inertial_scaling) :
-                0.5 * minl / sqrt(bulkm0 / rho0);
-            dt = dmin(dmin(dt_elastic, dt_maxwell), dmin(dt_advection, dt_diffusion)) * P->dt_fraction;
-            if (!(dt > 0)) set_error(s.sc, DES_ERR_RUNTIME);
-        }
-        s.sc->dt = dt;
+        const int e0 = b0 * BLOCK;
+        int c[Dim<ND>::NPE];
+        load_conn<ND>(s, e0, c);
+        const double bulkm0 = harmonic_mean(s, s.P->bulk_modulus, e0);
+        const double rho0 = mat_rho(s, e0, elem_mean<ND>(s.T, c));
+        store_dt(s.P, s.sc, sm[0][0], sm[1][0], sm[2][0], bulkm0, rho0);
     }
 }
 
@@ -814,29 +866,29 @@ template <int ND> __global__ void k_elem0_props(DevState s, double *out)
 }
 
 // ---- worst_elem_quality (geometry.cxx:458-512)
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial(DevState s)
+template <int ND> __device__ __forceinline__ double elem_quality(const DevState &s, int e)
 {
     constexpr int NPE = Dim<ND>::NPE;
-    double q = DBL_MAX;
-    int qi = 0x7fffffff;
-    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) {
-        int c[NPE];
-        load_conn<ND>(s, e, c);
-        double x[NPE][ND];
-        gather_vec<ND>(s.coord, s.N, c, x);
-        double vol = s.volume[e], quality;
-        if constexpr (ND == 3) {
-            double normalization_factor = 216 * sqrt(3.0);
-            double area_sum = (triangle_area<3>(x[0], x[1], x[2]) + triangle_area<3>(x[0], x[1], x[3]) +
-                               triangle_area<3>(x[2], x[3], x[0]) + triangle_area<3>(x[2], x[3], x[1]));
-            quality = normalization_factor * vol * vol / (area_sum * area_sum * area_sum);
-        } else {
-            double normalization_factor = 4 * sqrt(3.0);
-            double dist2_sum = dist2<2>(x[0], x[1]) + dist2<2>(x[1], x[2]) + dist2<2>(x[0], x[2]);
-            quality = normalization_factor * vol / dist2_sum;
-        }
-        if (quality < q) { q = quality; qi = e; }
+    int c[NPE];
+    load_conn<ND>(s, e, c);
+    double x[NPE][ND];
+    gather_vec<ND>(s.coord, s.N, c, x);
+    const double vol = s.volume[e];
+    if constexpr (ND == 3) {
+        double normalization_factor = 216 * sqrt(3.0);
+        double area_sum = (triangle_area<3>(x[0], x[1], x[2]) + triangle_area<3>(x[0], x[1], x[3]) +
+                           triangle_area<3>(x[2], x[3], x[0]) + triangle_area<3>(x[2], x[3], x[1]));
+        return normalization_factor * vol * vol / (area_sum * area_sum * area_sum);
+    } else {
+        double normalization_factor = 4 * sqrt(3.0);
+        double dist2_sum = dist2<2>(x[0], x[1]) + dist2<2>(x[1], x[2]) + dist2<2>(x[0], x[2]);
+        return normalization_factor * vol / dist2_sum;
     }
+}
+
+// (quality, element) minimum over the block, smaller index on ties; slot b
+__device__ __forceinline__ void block_argmin_store(DevState &s, int slot, double q, int qi)
+{
     __shared__ double sq[BLOCK];
     __shared__ int si[BLOCK];
     sq[threadIdx.x] = q;
@@ -854,9 +906,34 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial(Dev
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        s.red[blockIdx.x] = sq[0];
-        s.redi[blockIdx.x] = si[0];
+        s.red[slot] = sq[0];
+        s.redi[slot] = si[0];
     }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial(DevState s)
+{
+    double q = DBL_MAX;
+    int qi = 0x7fffffff;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) {
+        const double quality = elem_quality<ND>(s, e);
+        if (quality < q) { q = quality; qi = e; }
+    }
+    block_argmin_store(s, blockIdx.x, q, qi);
+}
+
+// batched models: one slot per element block (blocks never mix models)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial_b(DevState s)
+{
+    const int blk = xcd_block();
+    const int e = blk * BLOCK + threadIdx.x;
+    double q = DBL_MAX;
+    int qi = 0x7fffffff;
+    if (e < s.eblk_lim[blk]) {
+        q = elem_quality<ND>(s, e);
+        qi = e;
+    }
+    block_argmin_store(s, blk, q, qi);
 }
 
 // per-element fields of an output frame (output.cxx:147-166): MatProps::rho,
@@ -873,19 +950,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_elem_diagnostics(De
     double sr[NSTR];
     load_tensor<ND>(s.strain_rate, s.E, e, sr);
     viscosity[e] = mat_visc<ND>(s, e, Tm, sr);
-    double x[NPE][ND];
-    gather_vec<ND>(s.coord, s.N, c, x);
-    const double vol = s.volume[e];
-    if constexpr (ND == 3) {
-        double normalization_factor = 216 * sqrt(3.0);
-        double area_sum = (triangle_area<3>(x[0], x[1], x[2]) + triangle_area<3>(x[0], x[1], x[3]) +
-                           triangle_area<3>(x[2], x[3], x[0]) + triangle_area<3>(x[2], x[3], x[1]));
-        quality[e] = normalization_factor * vol * vol / (area_sum * area_sum * area_sum);
-    } else {
-        double normalization_factor = 4 * sqrt(3.0);
-        double dist2_sum = dist2<2>(x[0], x[1]) + dist2<2>(x[1], x[2]) + dist2<2>(x[0], x[2]);
-        quality[e] = normalization_factor * vol / dist2_sum;
-    }
+    quality[e] = elem_quality<ND>(s, e);
 }
 
 // ---- bad_mesh_quality's first two checks (remeshing.cxx:1893-1918): the
@@ -1284,6 +1349,24 @@ __global__ void k_step_end(DevState s, int rotate)
     sc->rot_pending = rotate;
 }
 
+// the same for every batched model, each with its own rheology
+__global__ void k_step_end_b(DevState s)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= s.nmodels) return;
+    DevScalars *sc = s.scb + m;
+    sc->time += sc->dt;
+    sc->steps += 1;
+    sc->dt_prev = sc->dt;
+    sc->rot_pending = (s.Pb[m].rheol_type & DES_RH_ELASTIC) ? 1 : 0;
+}
+
+__global__ void k_clear_rotation_b(DevState s)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < s.nmodels) s.scb[m].rot_pending = 0;
+}
+
 __global__ void k_clear_rotation(DevState s)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -1319,6 +1402,22 @@ struct des_ctx {
     std::vector<unsigned> h_bcflag;
     int passc_variant = -1;        // waves-per-EU variant of pass C chosen by the autotuner
     double passc_ms[4] = {0, 0, 0, 0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tune;  // autotuning launches in flight
+    // batched group (nmodels > 0): models of one dimension concatenated, each
+    // starting on a block boundary of the element and of the node arrays
+    int nmodels = 0;
+    std::vector<des_params> hP;
+    std::vector<int> e_off, n_off, e_cnt, n_cnt, nbf_cnt;
+    std::vector<char> has_visc, has_coh;  // per model: per-element viscosity / cohesion uploaded
+};
+
+// host description of a batched group's concatenated mesh
+struct GroupLayout {
+    std::vector<des_params> P;
+    std::vector<int> e_off, n_off, e_cnt, n_cnt, nbf_cnt;
+    std::vector<int> facet_model;          // model of each boundary facet
+    std::vector<double> bn, ev;            // per model 10 x nd, 10 x 10 x nd
+    std::vector<int> evv;                  // per model 10 x 10
 };
 
 namespace {
@@ -1376,22 +1475,29 @@ bool facet_loaded(const des_params &P, int i)
 }
 
 // host AoS [n][k] <-> device SoA [k][n]
-int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k)
+// host AoS [n][k] <-> device SoA [k][stride], rows off .. off+n-1
+int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0)
 {
     if (!src) return DES_OK;
+    if (stride < 0) stride = n;
     std::vector<double> tmp((size_t)n * k);
     for (long i = 0; i < n; ++i)
         for (int j = 0; j < k; ++j) tmp[(size_t)j * n + i] = src[(size_t)i * k + j];
-    HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    for (int j = 0; j < k; ++j)
+        HIPCHK(hipMemcpyAsync(dst + j * stride + off, tmp.data() + (size_t)j * n, n * sizeof(double),
+                              hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
 }
 
-int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k)
+int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0)
 {
     if (!dst) return DES_OK;
+    if (stride < 0) stride = n;
     std::vector<double> tmp((size_t)n * k);
-    HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    for (int j = 0; j < k; ++j)
+        HIPCHK(hipMemcpyAsync(tmp.data() + (size_t)j * n, src + j * stride + off, n * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (long i = 0; i < n; ++i)
         for (int j = 0; j < k; ++j) dst[(size_t)i * k + j] = tmp[(size_t)j * n + i];
@@ -1414,6 +1520,11 @@ void swap_volume(des_ctx *ctx)
 
 int launch_compute_dt(des_ctx *ctx)
 {
+    if (ctx->nmodels) {          // per model, entirely on the device
+        LAUNCH_ND(k_dt_partial_b, nblocks(ctx->E), ctx->s);
+        LAUNCH_ND(k_dt_final_b, ctx->nmodels, ctx->s);
+        return DES_OK;
+    }
     if (ctx->P.fixed_dt == 0) LAUNCH_ND(k_dt_partial, RED_BLOCKS, ctx->s);
     double bulkm0 = 0, rho0 = 1;
     if (!ctx->P.is_quasi_static && ctx->P.fixed_dt == 0) {
@@ -1465,8 +1576,13 @@ int flush_deferred(des_ctx *ctx)
     int rc = launch_mass(ctx);
     if (rc) return rc;
     LAUNCH_ND(k_shape_fn, nblocks(ctx->E), ctx->s);
-    if (ctx->P.rheol_type & DES_RH_ELASTIC) LAUNCH_ND(k_rotate, nblocks(ctx->E), ctx->s, 1);
-    hipLaunchKernelGGL(k_clear_rotation, dim3(1), dim3(1), 0, ctx->stream, ctx->s);
+    if (ctx->nmodels) {          // k_rotate skips the models without elasticity
+        LAUNCH_ND(k_rotate, nblocks(ctx->E), ctx->s, 1);
+        hipLaunchKernelGGL(k_clear_rotation_b, dim3(nblocks(ctx->nmodels)), dim3(BLOCK), 0, ctx->stream, ctx->s);
+    } else {
+        if (ctx->P.rheol_type & DES_RH_ELASTIC) LAUNCH_ND(k_rotate, nblocks(ctx->E), ctx->s, 1);
+        hipLaunchKernelGGL(k_clear_rotation, dim3(1), dim3(1), 0, ctx->stream, ctx->s);
+    }
     ctx->closing_pending = false;
     return check_launch(ctx);
 }
@@ -1475,110 +1591,99 @@ int flush_deferred(des_ctx *ctx)
 
 // ------------------------------------------------------------------ C-ABI
 
-extern "C" {
+namespace {
 
-const char *des_last_error(const des_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
-
-int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
+// One context: a single model (g == nullptr) or a batched group whose models
+// are concatenated into the mesh m, laid out as g describes.
+int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const GroupLayout *g)
 {
-    if (!p || !m || !out) return DES_ERR_INPUT;
-    *out = nullptr;
-    if (p->ndims != 2 && p->ndims != 3) return DES_ERR_INPUT;
-    if (p->nmat < 1 || p->nmat > DES_MAX_MAT) return DES_ERR_INPUT;
-    if (m->nnode <= 0 || m->nelem <= 0 || !m->connectivity || !m->coord || !m->bcflag) return DES_ERR_MESH;
-    des_ctx *ctx = new des_ctx();
     ctx->P = *p;
     const int nd = ctx->nd = p->ndims;
     const int npe = ctx->npe = nd + 1;
     ctx->nstr = nd == 3 ? 6 : 3;
     const int N = ctx->N = m->nnode, E = ctx->E = m->nelem;
     ctx->nbf = m->nbfacet;
-    // validate the mesh on the host: every index the kernels will follow
-    for (long k = 0; k < (long)E * npe; ++k)
-        if (m->connectivity[k] < 0 || m->connectivity[k] >= N) {
-            delete ctx;
-            return DES_ERR_MESH;
-        }
-    for (int k = 0; k < m->nbfacet; ++k)
-        if (m->bfacet_bdry[k] < 0 || m->bfacet_bdry[k] >= DES_NBDRY || m->bfacet_elem[k] < 0 ||
-            m->bfacet_elem[k] >= E || m->bfacet_face[k] < 0 || m->bfacet_face[k] > nd) {
-            delete ctx;
-            return DES_ERR_MESH;
-        }
-    if (!m->elemmarkers && p->nmat != 1) {
-        delete ctx;
-        return DES_ERR_INPUT;
-    }
-    {
-        hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) {
-            int rc = fail(ctx, DES_ERR_HIP, hipGetErrorString(e));
-            *out = ctx;
-            return rc;
-        }
-    }
-    *out = ctx;
+    const int nmodels = g ? (int)g->P.size() : 0;
+    ctx->nmodels = nmodels;
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     DevState &s = ctx->s;
     s.N = N;
     s.E = E;
     s.nbf = m->nbfacet;
     s.mpe = m->markers_per_element;
+    s.nmodels = nmodels;
     int rc;
 #define ALLOC(ptr, n) do { if ((rc = dalloc_n(ctx, &(ptr), (size_t)(n)))) return rc; } while (0)
+#define H2D(dst, src, n) HIPCHK(hipMemcpyAsync((void *)(dst), (src), sizeof(*(src)) * (size_t)(n), hipMemcpyHostToDevice, ctx->stream))
     des_params *dP;
-    ALLOC(dP, 1);
-    HIPCHK(hipMemcpyAsync(dP, p, sizeof(des_params), hipMemcpyHostToDevice, ctx->stream));
+    ALLOC(dP, nmodels ? nmodels : 1);
+    if (nmodels) H2D(dP, g->P.data(), nmodels);
+    else H2D(dP, p, 1);
     s.P = dP;
+    s.Pb = dP;
     int *conn;
     ALLOC(conn, (size_t)E * npe);
-    HIPCHK(hipMemcpyAsync(conn, m->connectivity, sizeof(int) * E * npe, hipMemcpyHostToDevice, ctx->stream));
+    H2D(conn, m->connectivity, (size_t)E * npe);
     s.conn = conn;
     unsigned *bcflag;
     ALLOC(bcflag, N);
-    HIPCHK(hipMemcpyAsync(bcflag, m->bcflag, sizeof(unsigned) * N, hipMemcpyHostToDevice, ctx->stream));
+    H2D(bcflag, m->bcflag, N);
     s.bcflag = bcflag;
     if (m->elemmarkers) {
         int *em;
         ALLOC(em, (size_t)E * p->nmat);
-        HIPCHK(hipMemcpyAsync(em, m->elemmarkers, sizeof(int) * E * p->nmat, hipMemcpyHostToDevice, ctx->stream));
+        H2D(em, m->elemmarkers, (size_t)E * p->nmat);
         s.elemmarkers = em;
     }
+    // live elements: a batched group's padding between models is not
+    std::vector<char> live(E, 1);
+    if (nmodels)
+        for (int k = 0; k < nmodels; ++k)
+            for (int e = g->e_off[k] + g->e_cnt[k]; e < (k + 1 < nmodels ? g->e_off[k + 1] : E); ++e) live[e] = 0;
     // create_support (mesh.cxx:1707-1721) as a CSR, ascending elements per node
-    std::vector<int> off(N + 1, 0), ent((size_t)E * npe);
-    for (long k = 0; k < (long)E * npe; ++k) off[m->connectivity[k] + 1]++;
+    std::vector<int> off(N + 1, 0);
+    for (int e = 0; e < E; ++e)
+        if (live[e])
+            for (int i = 0; i < npe; ++i) off[m->connectivity[(size_t)e * npe + i] + 1]++;
     for (int n = 0; n < N; ++n) off[n + 1] += off[n];
+    std::vector<int> ent(std::max(off[N], 1));
     {
         std::vector<int> fill(off.begin(), off.end() - 1);
         for (int e = 0; e < E; ++e)
-            for (int i = 0; i < npe; ++i) ent[fill[m->connectivity[(size_t)e * npe + i]]++] = e * npe + i;
+            if (live[e])
+                for (int i = 0; i < npe; ++i) ent[fill[m->connectivity[(size_t)e * npe + i]]++] = e * npe + i;
     }
     int *d_off, *d_ent;
     ALLOC(d_off, N + 1);
-    ALLOC(d_ent, (size_t)E * npe);
-    HIPCHK(hipMemcpyAsync(d_off, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(d_ent, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice, ctx->stream));
+    ALLOC(d_ent, ent.size());
+    H2D(d_off, off.data(), N + 1);
+    H2D(d_ent, ent.data(), ent.size());
     s.csr_off = d_off;
     s.csr_ent = d_ent;
     // boundary facets; the loaded ones get a per-node list in reference order
+    // (boundary by boundary, facets ascending)
     if (m->nbfacet > 0) {
         int *b, *e, *f;
         ALLOC(b, m->nbfacet);
         ALLOC(e, m->nbfacet);
         ALLOC(f, m->nbfacet);
-        HIPCHK(hipMemcpyAsync(b, m->bfacet_bdry, sizeof(int) * m->nbfacet, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(e, m->bfacet_elem, sizeof(int) * m->nbfacet, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(f, m->bfacet_face, sizeof(int) * m->nbfacet, hipMemcpyHostToDevice, ctx->stream));
+        H2D(b, m->bfacet_bdry, m->nbfacet);
+        H2D(e, m->bfacet_elem, m->nbfacet);
+        H2D(f, m->bfacet_face, m->nbfacet);
         s.bf_bdry = b;
         s.bf_elem = e;
         s.bf_face = f;
         static const int NOF3[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
         static const int NOF2[3][2] = {{1, 2}, {2, 0}, {0, 1}};
-        std::vector<int> act;
-        for (int i = 0; i < DES_NBDRY; ++i) {
-            if (!facet_loaded(*p, i)) continue;
-            for (int k = 0; k < m->nbfacet; ++k)
-                if (m->bfacet_bdry[k] == i) act.push_back(k);
-        }
+        std::vector<int> act, act_model;
+        for (int i = 0; i < DES_NBDRY; ++i)
+            for (int k = 0; k < m->nbfacet; ++k) {
+                if (m->bfacet_bdry[k] != i) continue;
+                const int mk = nmodels ? g->facet_model[k] : 0;
+                if (!facet_loaded(nmodels ? g->P[mk] : *p, i)) continue;
+                act.push_back(k);
+                act_model.push_back(mk);
+            }
         if (!act.empty()) {
             std::vector<int> foff(N + 1, 0);
             auto facet_node = [&](int k, int j) {
@@ -1596,30 +1701,40 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
             ALLOC(d_foff, N + 1);
             ALLOC(d_fent, fent.size());
             ALLOC(d_act, act.size());
-            HIPCHK(hipMemcpyAsync(d_foff, foff.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(d_fent, fent.data(), sizeof(int) * fent.size(), hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(d_act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, ctx->stream));
+            H2D(d_foff, foff.data(), N + 1);
+            H2D(d_fent, fent.data(), fent.size());
+            H2D(d_act, act.data(), act.size());
             s.ff_off = d_foff;
             s.ff_ent = d_fent;
             ctx->active_facets = d_act;
             ctx->nactive = (int)act.size();
             ALLOC(s.ftrac, (size_t)m->nbfacet * nd);
+            if (nmodels) {
+                int *fm;
+                ALLOC(fm, act_model.size());
+                H2D(fm, act_model.data(), act_model.size());
+                s.fmodel = fm;
+            }
         }
     }
     double *bn, *ev;
     int *evv;
-    ALLOC(bn, DES_NBDRY * nd);
-    ALLOC(ev, DES_NBDRY * DES_NBDRY * nd);
-    ALLOC(evv, DES_NBDRY * DES_NBDRY);
-    if (m->bnormals)
-        HIPCHK(hipMemcpyAsync(bn, m->bnormals, sizeof(double) * DES_NBDRY * nd, hipMemcpyHostToDevice, ctx->stream));
-    if (m->edge_vectors)
-        HIPCHK(hipMemcpyAsync(ev, m->edge_vectors, sizeof(double) * DES_NBDRY * DES_NBDRY * nd, hipMemcpyHostToDevice, ctx->stream));
-    if (m->edge_valid)
-        HIPCHK(hipMemcpyAsync(evv, m->edge_valid, sizeof(int) * DES_NBDRY * DES_NBDRY, hipMemcpyHostToDevice, ctx->stream));
-    s.bnormals = bn;
-    s.edge_vectors = ev;
-    s.edge_valid = evv;
+    const int ngeo = nmodels ? nmodels : 1;
+    ALLOC(bn, (size_t)ngeo * DES_NBDRY * nd);
+    ALLOC(ev, (size_t)ngeo * DES_NBDRY * DES_NBDRY * nd);
+    ALLOC(evv, (size_t)ngeo * DES_NBDRY * DES_NBDRY);
+    if (nmodels) {
+        H2D(bn, g->bn.data(), g->bn.size());
+        H2D(ev, g->ev.data(), g->ev.size());
+        H2D(evv, g->evv.data(), g->evv.size());
+    } else {
+        if (m->bnormals) H2D(bn, m->bnormals, DES_NBDRY * nd);
+        if (m->edge_vectors) H2D(ev, m->edge_vectors, DES_NBDRY * DES_NBDRY * nd);
+        if (m->edge_valid) H2D(evv, m->edge_valid, DES_NBDRY * DES_NBDRY);
+    }
+    s.bnormals = s.bnb = bn;
+    s.edge_vectors = s.evb = ev;
+    s.edge_valid = s.evvb = evv;
     // fields (zero, as allocate_variables leaves them)
     ALLOC(s.coord, (size_t)N * nd);
     ALLOC(s.coord0, (size_t)N * nd);
@@ -1646,18 +1761,92 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
     ALLOC(s.ework, (size_t)E * npe * nd);
     ALLOC(s.escal, (size_t)E * 4);
     ALLOC(s.erec, (size_t)E * 8);
-    ALLOC(s.sc, 1);
-    ALLOC(s.red, RED_BLOCKS * 3);
-    ALLOC(s.redi, RED_BLOCKS);
+    ALLOC(s.sc, ngeo);
+    s.scb = s.sc;
+    const int nred = std::max(RED_BLOCKS, nblocks(E));
+    ALLOC(s.red, (size_t)nred * 3);
+    ALLOC(s.redi, (size_t)nred * 2);
     ALLOC(ctx->elem0, 2);
+    if (nmodels) {
+        // block -> model tables; a block's real elements (nodes) end at lim
+        const int neb = nblocks(E), nnb = nblocks(N);
+        std::vector<int> ebm(neb, 0), ebl(neb, 0), nbm(nnb, 0), nbl(nnb, 0), mblk(nmodels + 1, neb);
+        for (int k = 0; k < nmodels; ++k) {
+            const int e0 = g->e_off[k], e1 = e0 + g->e_cnt[k], n0 = g->n_off[k], n1 = n0 + g->n_cnt[k];
+            mblk[k] = e0 / BLOCK;
+            for (int b = e0 / BLOCK; b < nblocks(e1); ++b) { ebm[b] = k; ebl[b] = e1; }
+            for (int b = n0 / BLOCK; b < nblocks(n1); ++b) { nbm[b] = k; nbl[b] = n1; }
+        }
+        int *d_ebm, *d_ebl, *d_nbm, *d_nbl, *d_mblk;
+        ALLOC(d_ebm, neb);
+        ALLOC(d_ebl, neb);
+        ALLOC(d_nbm, nnb);
+        ALLOC(d_nbl, nnb);
+        ALLOC(d_mblk, nmodels + 1);
+        H2D(d_ebm, ebm.data(), neb);
+        H2D(d_ebl, ebl.data(), neb);
+        H2D(d_nbm, nbm.data(), nnb);
+        H2D(d_nbl, nbl.data(), nnb);
+        H2D(d_mblk, mblk.data(), nmodels + 1);
+        s.eblk_model = d_ebm;
+        s.eblk_lim = d_ebl;
+        s.nblk_model = d_nbm;
+        s.nblk_lim = d_nbl;
+        s.mblk_off = d_mblk;
+        ctx->hP = g->P;
+        ctx->e_off = g->e_off;
+        ctx->n_off = g->n_off;
+        ctx->e_cnt = g->e_cnt;
+        ctx->n_cnt = g->n_cnt;
+        ctx->nbf_cnt = g->nbf_cnt;
+        ctx->has_visc.assign(nmodels, 0);
+        ctx->has_coh.assign(nmodels, 0);
+    } else {
+        ctx->h_conn.assign(m->connectivity, m->connectivity + (size_t)E * npe);
+        ctx->h_bcflag.assign(m->bcflag, m->bcflag + N);
+    }
 #undef ALLOC
-    ctx->h_conn.assign(m->connectivity, m->connectivity + (size_t)E * npe);
-    ctx->h_bcflag.assign(m->bcflag, m->bcflag + N);
+#undef H2D
     // initial coordinates, also coord0 (dynearthsol.cxx:82-84)
     if ((rc = upload_soa(ctx, s.coord, m->coord, N, nd))) return rc;
     if ((rc = upload_soa(ctx, s.coord0, m->coord, N, nd))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
+}
+
+// every index the kernels will follow, checked on the host
+int validate_mesh(const des_params *p, const des_mesh *m)
+{
+    if (p->ndims != 2 && p->ndims != 3) return DES_ERR_INPUT;
+    if (p->nmat < 1 || p->nmat > DES_MAX_MAT) return DES_ERR_INPUT;
+    if (m->nnode <= 0 || m->nelem <= 0 || !m->connectivity || !m->coord || !m->bcflag) return DES_ERR_MESH;
+    const int nd = p->ndims, npe = nd + 1;
+    for (long k = 0; k < (long)m->nelem * npe; ++k)
+        if (m->connectivity[k] < 0 || m->connectivity[k] >= m->nnode) return DES_ERR_MESH;
+    if (m->nbfacet > 0 && (!m->bfacet_bdry || !m->bfacet_elem || !m->bfacet_face)) return DES_ERR_MESH;
+    for (int k = 0; k < m->nbfacet; ++k)
+        if (m->bfacet_bdry[k] < 0 || m->bfacet_bdry[k] >= DES_NBDRY || m->bfacet_elem[k] < 0 ||
+            m->bfacet_elem[k] >= m->nelem || m->bfacet_face[k] < 0 || m->bfacet_face[k] > nd)
+            return DES_ERR_MESH;
+    if (!m->elemmarkers && p->nmat != 1) return DES_ERR_INPUT;
+    return DES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *des_last_error(const des_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
+{
+    if (!p || !m || !out) return DES_ERR_INPUT;
+    *out = nullptr;
+    int rc = validate_mesh(p, m);
+    if (rc) return rc;
+    des_ctx *ctx = new des_ctx();
+    *out = ctx;
+    return create_impl(ctx, p, m, nullptr);
 }
 
 void des_destroy(des_ctx *ctx)
@@ -1669,74 +1858,86 @@ void des_destroy(des_ctx *ctx)
     delete ctx;
 }
 
-static int field_opt(des_ctx *ctx, double **slot, const double *src, long n)
+}  // extern "C"
+
+namespace {
+
+// per-element optional field, allocated on first upload
+int field_opt(des_ctx *ctx, double **slot, const double *src, long n, long off)
 {
     if (!src) return DES_OK;
     if (!*slot) {
-        int rc = dalloc_n(ctx, slot, (size_t)n);
+        int rc = dalloc_n(ctx, slot, (size_t)ctx->E);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(*slot, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(*slot + off, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     return DES_OK;
 }
+
+// The fields of des_fields between host AoS arrays and the device SoA state:
+// the whole context (model < 0) or batched model `model` of a group.
+int xfer_fields(des_ctx *ctx, des_fields *f, bool up, int model)
+{
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    DevState &s = ctx->s;
+    const long NS = ctx->N, ES = ctx->E;
+    const long N = model < 0 ? NS : ctx->n_cnt[model], E = model < 0 ? ES : ctx->e_cnt[model];
+    const long no = model < 0 ? 0 : ctx->n_off[model], eo = model < 0 ? 0 : ctx->e_off[model];
+    const int nd = ctx->nd, npe = ctx->npe, nstr = ctx->nstr;
+    struct { double *dev; double *host; long n; int k; bool elem; } tab[] = {
+        {s.coord, f->coord, N, nd, false}, {s.coord0, f->coord0, N, nd, false},
+        {s.vel, f->vel, N, nd, false}, {s.force, f->force, N, nd, false},
+        {s.T, f->temperature, N, 1, false}, {s.mass, f->mass, N, 1, false},
+        {s.tmass, f->tmass, N, 1, false}, {s.volume_n, f->volume_n, N, 1, false},
+        {s.ntmp, f->ntmp, N, 1, false},
+        {s.volume, f->volume, E, 1, true}, {s.volume_old, f->volume_old, E, 1, true},
+        {s.edvoldt, f->edvoldt, E, 1, true}, {s.plstrain, f->plstrain, E, 1, true},
+        {s.delta_plstrain, f->delta_plstrain, E, 1, true}, {s.dpressure, f->dpressure, E, 1, true},
+        {s.stressyy, f->stressyy, E, 1, true}, {s.strain_rate, f->strain_rate, E, nstr, true},
+        {s.strain, f->strain, E, nstr, true}, {s.stress, f->stress, E, nstr, true},
+    };
+    for (auto &t : tab) {
+        const long stride = t.elem ? ES : NS, off = t.elem ? eo : no;
+        rc = up ? upload_soa(ctx, t.dev, t.host, t.n, t.k, stride, off)
+                : download_soa(ctx, t.host, t.dev, t.n, t.k, stride, off);
+        if (rc) return rc;
+    }
+    double *shp_host[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
+    for (int d = 0; d < nd; ++d) {
+        double *dev = s.shp + (size_t)d * npe * ES;
+        rc = up ? upload_soa(ctx, dev, shp_host[d], E, npe, ES, eo) : download_soa(ctx, shp_host[d], dev, E, npe, ES, eo);
+        if (rc) return rc;
+    }
+    if (up) {
+        double *ev = const_cast<double *>(s.elem_visc), *ec = const_cast<double *>(s.elem_cohesion);
+        if ((rc = field_opt(ctx, &ev, f->elem_visc, E, eo))) return rc;
+        if ((rc = field_opt(ctx, &ec, f->elem_cohesion, E, eo))) return rc;
+        s.elem_visc = ev;
+        s.elem_cohesion = ec;
+        if (model >= 0) {
+            if (f->elem_visc) ctx->has_visc[model] = 1;
+            if (f->elem_cohesion) ctx->has_coh[model] = 1;
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return DES_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int des_upload_fields(des_ctx *ctx, const des_fields *f)
 {
     if (!ctx || !f) return DES_ERR_INPUT;
-    int rc = flush_deferred(ctx);
-    if (rc) return rc;
-    DevState &s = ctx->s;
-    const long N = ctx->N, E = ctx->E;
-    const int nd = ctx->nd, npe = ctx->npe, nstr = ctx->nstr;
-    struct { double *dst; const double *src; long n; int k; } tab[] = {
-        {s.coord, f->coord, N, nd}, {s.coord0, f->coord0, N, nd}, {s.vel, f->vel, N, nd},
-        {s.force, f->force, N, nd}, {s.T, f->temperature, N, 1}, {s.mass, f->mass, N, 1},
-        {s.tmass, f->tmass, N, 1}, {s.volume_n, f->volume_n, N, 1}, {s.ntmp, f->ntmp, N, 1},
-        {s.volume, f->volume, E, 1}, {s.volume_old, f->volume_old, E, 1},
-        {s.edvoldt, f->edvoldt, E, 1}, {s.plstrain, f->plstrain, E, 1},
-        {s.delta_plstrain, f->delta_plstrain, E, 1}, {s.dpressure, f->dpressure, E, 1},
-        {s.stressyy, f->stressyy, E, 1}, {s.strain_rate, f->strain_rate, E, nstr},
-        {s.strain, f->strain, E, nstr}, {s.stress, f->stress, E, nstr},
-    };
-    for (auto &t : tab)
-        if ((rc = upload_soa(ctx, t.dst, t.src, t.n, t.k))) return rc;
-    const double *shp_src[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
-    for (int d = 0; d < nd; ++d)
-        if (shp_src[d])
-            if ((rc = upload_soa(ctx, s.shp + (size_t)d * npe * E, shp_src[d], E, npe))) return rc;
-    double *ev = const_cast<double *>(s.elem_visc), *ec = const_cast<double *>(s.elem_cohesion);
-    if ((rc = field_opt(ctx, &ev, f->elem_visc, E))) return rc;
-    if ((rc = field_opt(ctx, &ec, f->elem_cohesion, E))) return rc;
-    s.elem_visc = ev;
-    s.elem_cohesion = ec;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    return DES_OK;
+    return xfer_fields(ctx, const_cast<des_fields *>(f), true, -1);
 }
 
 int des_download_fields(des_ctx *ctx, des_fields *f)
 {
     if (!ctx || !f) return DES_ERR_INPUT;
-    int rc = flush_deferred(ctx);
-    if (rc) return rc;
-    DevState &s = ctx->s;
-    const long N = ctx->N, E = ctx->E;
-    const int nd = ctx->nd, npe = ctx->npe, nstr = ctx->nstr;
-    struct { double *dst; const double *src; long n; int k; } tab[] = {
-        {f->coord, s.coord, N, nd}, {f->coord0, s.coord0, N, nd}, {f->vel, s.vel, N, nd},
-        {f->force, s.force, N, nd}, {f->temperature, s.T, N, 1}, {f->mass, s.mass, N, 1},
-        {f->tmass, s.tmass, N, 1}, {f->volume_n, s.volume_n, N, 1}, {f->ntmp, s.ntmp, N, 1},
-        {f->volume, s.volume, E, 1}, {f->volume_old, s.volume_old, E, 1},
-        {f->edvoldt, s.edvoldt, E, 1}, {f->plstrain, s.plstrain, E, 1},
-        {f->delta_plstrain, s.delta_plstrain, E, 1}, {f->dpressure, s.dpressure, E, 1},
-        {f->stressyy, s.stressyy, E, 1}, {f->strain_rate, s.strain_rate, E, nstr},
-        {f->strain, s.strain, E, nstr}, {f->stress, s.stress, E, nstr},
-    };
-    for (auto &t : tab)
-        if ((rc = download_soa(ctx, t.dst, t.src, t.n, t.k))) return rc;
-    double *shp_dst[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
-    for (int d = 0; d < nd; ++d)
-        if ((rc = download_soa(ctx, shp_dst[d], s.shp + (size_t)d * npe * E, E, npe))) return rc;
-    return DES_OK;
+    return xfer_fields(ctx, f, false, -1);
 }
 
 // One output frame in the reference's binary format (binaryio.cxx:16-36: a
@@ -2118,88 +2319,123 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
     return check_launch(ctx);
 }
 
-int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
+int fused_begin(des_ctx *ctx, int precision)
 {
     if (precision != DES_FP64 && precision != DES_MIXED)
         return fail(ctx, DES_ERR_INPUT, "unknown precision");
-    if (precision == DES_MIXED && ctx->P.is_plane_strain)
-        return fail(ctx, DES_ERR_INPUT, "mixed precision does not cover plane strain");
-    if (nsteps <= 0) return DES_OK;
+    const int nm = std::max(ctx->nmodels, 1);
+    for (int k = 0; k < nm; ++k) {
+        const des_params &P = ctx->nmodels ? ctx->hP[k] : ctx->P;
+        if (precision == DES_MIXED && P.is_plane_strain)
+            return fail(ctx, DES_ERR_INPUT, "mixed precision does not cover plane strain");
+    }
+    if (ctx->nmodels) {
+        // per-element viscosity / cohesion are read for every model or none
+        for (int k = 1; k < ctx->nmodels; ++k)
+            if (ctx->has_visc[k] != ctx->has_visc[0] || ctx->has_coh[k] != ctx->has_coh[0])
+                return fail(ctx, DES_ERR_INPUT, "per-element viscosity / cohesion uploaded for some models only");
+    }
     int rc = flush_deferred(ctx);
     if (rc) return rc;
-    const bool mixed = precision == DES_MIXED;
-    if (mixed) {
+    if (precision == DES_MIXED) {
         if ((rc = ensure_mixed(ctx))) return rc;
         LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);
     }
-    const int rotate = (ctx->P.rheol_type & DES_RH_ELASTIC) ? 1 : 0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> tune;
-    MARK(-1);
-    for (int k = 0; k < nsteps; ++k) {
-        FusedFlags fl;
-        fl.use_stored_mass = (k == 0);
-        fl.store_aux = (k == nsteps - 1);
-        if (k > 0) swap_volume(ctx);
-        LAUNCH_ND(k_pass_a, nblocks(ctx->E), ctx->s, fl);
-        MARK(0);
-        LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
-        MARK(1);
-        if (mixed) {
-            LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
-        } else if (ctx->passc_variant < 0 && tune.size() < 2 * PASSC_VARIANTS && nsteps >= 16) {
-            // autotuning: the first 2 x 4 steps of a long enough run time each variant twice
-            const int var = (int)tune.size() % PASSC_VARIANTS;
-            hipEvent_t a, b;
-            HIPCHK(hipEventCreate(&a));
-            HIPCHK(hipEventCreate(&b));
-            HIPCHK(hipEventRecord(a, ctx->stream));
-            if ((rc = launch_pass_c(ctx, fl, var))) return rc;
-            HIPCHK(hipEventRecord(b, ctx->stream));
-            tune.push_back({a, b});
-            if (tune.size() == 2 * PASSC_VARIANTS) {
-                HIPCHK(hipEventSynchronize(b));
-                double best = 1e300;
-                for (int v = 0; v < PASSC_VARIANTS; ++v) ctx->passc_ms[v] = 0;
-                for (size_t k = 0; k < tune.size(); ++k) {
-                    float t = 0;
-                    HIPCHK(hipEventElapsedTime(&t, tune[k].first, tune[k].second));
-                    ctx->passc_ms[k % PASSC_VARIANTS] += t;
-                    (void)hipEventDestroy(tune[k].first);
-                    (void)hipEventDestroy(tune[k].second);
-                }
-                for (int v = 0; v < PASSC_VARIANTS; ++v)
-                    if (ctx->passc_ms[v] < best) {
-                        best = ctx->passc_ms[v];
-                        ctx->passc_variant = v;
-                    }
+    return DES_OK;
+}
+
+// Step k of an nsteps run: the six passes, the step's bookkeeping and, every
+// tenth step, compute_dt.
+int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
+{
+    int rc;
+    FusedFlags fl;
+    fl.use_stored_mass = (k == 0);
+    fl.store_aux = (k == nsteps - 1);
+    if (k > 0) swap_volume(ctx);
+    LAUNCH_ND(k_pass_a, nblocks(ctx->E), ctx->s, fl);
+    MARK(0);
+    LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
+    MARK(1);
+    auto &tune = ctx->tune;
+    if (mixed) {
+        LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
+    } else if (ctx->passc_variant < 0 && tune.size() < 2 * PASSC_VARIANTS && nsteps >= 16) {
+        // autotuning: the first 2 x 4 steps of a long enough run time each variant twice
+        const int var = (int)tune.size() % PASSC_VARIANTS;
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, ctx->stream));
+        if ((rc = launch_pass_c(ctx, fl, var))) return rc;
+        HIPCHK(hipEventRecord(b, ctx->stream));
+        tune.push_back({a, b});
+        if (tune.size() == 2 * PASSC_VARIANTS) {
+            HIPCHK(hipEventSynchronize(b));
+            double best = 1e300;
+            for (int v = 0; v < PASSC_VARIANTS; ++v) ctx->passc_ms[v] = 0;
+            for (size_t j = 0; j < tune.size(); ++j) {
+                float t = 0;
+                HIPCHK(hipEventElapsedTime(&t, tune[j].first, tune[j].second));
+                ctx->passc_ms[j % PASSC_VARIANTS] += t;
+                (void)hipEventDestroy(tune[j].first);
+                (void)hipEventDestroy(tune[j].second);
             }
-        } else {
-            if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
+            tune.clear();
+            for (int v = 0; v < PASSC_VARIANTS; ++v)
+                if (ctx->passc_ms[v] < best) {
+                    best = ctx->passc_ms[v];
+                    ctx->passc_variant = v;
+                }
         }
-        MARK(2);
-        LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s);
-        MARK(3);
-        if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
-        else LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
-        MARK(4);
-        if (ctx->nactive) {
-            LAUNCH_ND(k_facet_traction, nblocks(ctx->nactive), ctx->s, ctx->active_facets, ctx->nactive);
-            MARK(5);
-        }
-        LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, fl);
-        MARK(6);
-        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->s, rotate);
-        if ((rc = check_launch(ctx))) return rc;
-        MARK(-1);
-        ctx->steps++;
-        ctx->closing_pending = true;
-        if (ctx->steps % 10 == 0) {
-            if ((rc = launch_compute_dt(ctx))) return rc;
-            MARK(7);
-        }
+    } else {
+        if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
     }
+    MARK(2);
+    LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s);
+    MARK(3);
+    if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
+    else LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
+    MARK(4);
+    if (ctx->nactive) {
+        LAUNCH_ND(k_facet_traction, nblocks(ctx->nactive), ctx->s, ctx->active_facets, ctx->nactive);
+        MARK(5);
+    }
+    LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, fl);
+    MARK(6);
+    if (ctx->nmodels) {
+        hipLaunchKernelGGL(k_step_end_b, dim3(nblocks(ctx->nmodels)), dim3(BLOCK), 0, ctx->stream, ctx->s);
+    } else {
+        const int rotate = (ctx->P.rheol_type & DES_RH_ELASTIC) ? 1 : 0;
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->s, rotate);
+    }
+    if ((rc = check_launch(ctx))) return rc;
+    MARK(-1);
+    ctx->steps++;
+    ctx->closing_pending = true;
+    if (ctx->steps % 10 == 0) {
+        if ((rc = launch_compute_dt(ctx))) return rc;
+        MARK(7);
+    }
+    return DES_OK;
+}
+
+int fused_end(des_ctx *ctx, bool mixed)
+{
     if (mixed) LAUNCH_ND(k_from_mixed, nblocks(ctx->E), ctx->s);
     return DES_OK;
+}
+
+int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
+{
+    if (nsteps <= 0) return precision == DES_FP64 || precision == DES_MIXED ? DES_OK : DES_ERR_INPUT;
+    int rc = fused_begin(ctx, precision);
+    if (rc) return rc;
+    const bool mixed = precision == DES_MIXED;
+    MARK(-1);
+    for (int k = 0; k < nsteps; ++k)
+        if ((rc = fused_step(ctx, k, nsteps, mixed, tm))) return rc;
+    return fused_end(ctx, mixed);
 }
 
 }  // namespace
@@ -2230,6 +2466,371 @@ int des_sync(des_ctx *ctx)
     int rc = read_scalars(ctx, h);
     if (rc) return rc;
     if (h.err) return fail(ctx, h.err, "device reported an error");
+    return DES_OK;
+}
+
+}  // extern "C"
+
+// ================================================================ batches
+//
+// Many independent models advance together.  Models of one dimension form a
+// group: their meshes are concatenated into one context, each model's
+// elements and nodes starting on a block boundary, so that every launch of
+// the fused schedule covers all models of the group and each block works for
+// exactly one model (its parameters and scalars are bound from the block ->
+// model tables).  The 2D and the 3D group run on their own streams, one step
+// of each enqueued in turn.
+
+struct des_batch {
+    std::vector<des_ctx *> groups;
+    std::vector<int> group_of, index_of;   // per model
+    std::vector<int> msteps;               // per model: var.steps as last set
+    std::string err;
+};
+
+namespace {
+
+int bfail(des_batch *b, int code, const std::string &msg)
+{
+    b->err = msg;
+    return code;
+}
+
+// a group call's status, with the group's message on failure
+int bcheck(des_batch *b, des_ctx *ctx, int rc)
+{
+    if (rc) b->err = ctx->err.empty() ? "status " + std::to_string(rc) : ctx->err;
+    return rc;
+}
+
+int build_group(des_batch *b, const std::vector<int> &ids, const des_params *params, const des_mesh *meshes)
+{
+    const des_params &P0 = params[ids[0]];
+    const des_mesh &M0 = meshes[ids[0]];
+    const int nd = P0.ndims, npe = nd + 1, nmat = P0.nmat;
+    GroupLayout g;
+    long E = 0, N = 0, F = 0;
+    for (int id : ids) {
+        const des_params &p = params[id];
+        const des_mesh &m = meshes[id];
+        if (p.nmat != nmat || m.markers_per_element != M0.markers_per_element)
+            return bfail(b, DES_ERR_INPUT, "models of one dimension must share nmat and markers_per_element");
+        g.P.push_back(p);
+        g.e_off.push_back((int)E);
+        g.n_off.push_back((int)N);
+        g.e_cnt.push_back(m.nelem);
+        g.n_cnt.push_back(m.nnode);
+        g.nbf_cnt.push_back(m.nbfacet);
+        E += (long)nblocks(m.nelem) * BLOCK;
+        N += (long)nblocks(m.nnode) * BLOCK;
+        F += m.nbfacet;
+        if (E * npe >= (1L << 31) || N >= (1L << 31))
+            return bfail(b, DES_ERR_INPUT, "batched group too large for 32-bit indices");
+    }
+    std::vector<int> conn((size_t)E * npe, 0), bdry, felem, fface;
+    std::vector<double> coord((size_t)N * nd, 0.0);
+    std::vector<unsigned> bcflag(N, 0u);
+    std::vector<int> markers(nmat > 1 ? (size_t)E * nmat : 0, 0);
+    bdry.reserve(F);
+    felem.reserve(F);
+    fface.reserve(F);
+    for (size_t k = 0; k < ids.size(); ++k) {
+        const des_mesh &m = meshes[ids[k]];
+        const int eo = g.e_off[k], no = g.n_off[k];
+        for (long i = 0; i < (long)m.nelem * npe; ++i) conn[(size_t)eo * npe + i] = m.connectivity[i] + no;
+        std::memcpy(coord.data() + (size_t)no * nd, m.coord, sizeof(double) * m.nnode * nd);
+        std::memcpy(bcflag.data() + no, m.bcflag, sizeof(unsigned) * m.nnode);
+        if (nmat > 1) std::memcpy(markers.data() + (size_t)eo * nmat, m.elemmarkers, sizeof(int) * m.nelem * nmat);
+        for (int f = 0; f < m.nbfacet; ++f) {
+            bdry.push_back(m.bfacet_bdry[f]);
+            felem.push_back(m.bfacet_elem[f] + eo);
+            fface.push_back(m.bfacet_face[f]);
+            g.facet_model.push_back((int)k);
+        }
+        const size_t nbn = DES_NBDRY * nd, nev = DES_NBDRY * DES_NBDRY * nd, nevv = DES_NBDRY * DES_NBDRY;
+        const size_t bn0 = g.bn.size(), ev0 = g.ev.size(), evv0 = g.evv.size();
+        g.bn.resize(bn0 + nbn, 0.0);
+        g.ev.resize(ev0 + nev, 0.0);
+        g.evv.resize(evv0 + nevv, 0);
+        if (m.bnormals) std::memcpy(g.bn.data() + bn0, m.bnormals, sizeof(double) * nbn);
+        if (m.edge_vectors) std::memcpy(g.ev.data() + ev0, m.edge_vectors, sizeof(double) * nev);
+        if (m.edge_valid) std::memcpy(g.evv.data() + evv0, m.edge_valid, sizeof(int) * nevv);
+    }
+    des_mesh sm{};
+    sm.nnode = (int)N;
+    sm.nelem = (int)E;
+    sm.connectivity = conn.data();
+    sm.coord = coord.data();
+    sm.bcflag = bcflag.data();
+    sm.nbfacet = (int)F;
+    sm.bfacet_bdry = bdry.data();
+    sm.bfacet_elem = felem.data();
+    sm.bfacet_face = fface.data();
+    sm.elemmarkers = nmat > 1 ? markers.data() : nullptr;
+    sm.markers_per_element = M0.markers_per_element;
+    des_ctx *ctx = new des_ctx();
+    b->groups.push_back(ctx);
+    int rc = create_impl(ctx, &P0, &sm, &g);
+    if (rc) return bcheck(b, ctx, rc);
+    for (size_t k = 0; k < ids.size(); ++k) {
+        b->group_of[ids[k]] = (int)b->groups.size() - 1;
+        b->index_of[ids[k]] = (int)k;
+    }
+    return DES_OK;
+}
+
+int model_ctx(des_batch *b, int model, des_ctx **ctx, int *idx)
+{
+    if (model < 0 || model >= (int)b->group_of.size()) return bfail(b, DES_ERR_INPUT, "model index out of range");
+    *ctx = b->groups[b->group_of[model]];
+    *idx = b->index_of[model];
+    return DES_OK;
+}
+
+int read_model_scalars(des_ctx *ctx, int idx, DevScalars &h)
+{
+    HIPCHK(hipMemcpyAsync(&h, ctx->s.scb + idx, sizeof(DevScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *des_batch_last_error(const des_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int des_batch_create(int nmodels, const des_params *params, const des_mesh *meshes, des_batch **out)
+{
+    if (!out) return DES_ERR_INPUT;
+    *out = nullptr;
+    if (nmodels <= 0 || !params || !meshes) return DES_ERR_INPUT;
+    for (int k = 0; k < nmodels; ++k) {
+        int rc = validate_mesh(&params[k], &meshes[k]);
+        if (rc) return rc;
+    }
+    des_batch *b = new des_batch();
+    *out = b;
+    b->group_of.assign(nmodels, -1);
+    b->index_of.assign(nmodels, -1);
+    b->msteps.assign(nmodels, 0);
+    for (int nd = 2; nd <= 3; ++nd) {
+        std::vector<int> ids;
+        for (int k = 0; k < nmodels; ++k)
+            if (params[k].ndims == nd) ids.push_back(k);
+        if (ids.empty()) continue;
+        int rc = build_group(b, ids, params, meshes);
+        if (rc) return rc;
+    }
+    return DES_OK;
+}
+
+void des_batch_destroy(des_batch *b)
+{
+    if (!b) return;
+    for (des_ctx *c : b->groups) des_destroy(c);
+    delete b;
+}
+
+int des_batch_upload_fields(des_batch *b, int model, const des_fields *f)
+{
+    if (!b || !f) return DES_ERR_INPUT;
+    des_ctx *ctx;
+    int idx, rc = model_ctx(b, model, &ctx, &idx);
+    if (rc) return rc;
+    return bcheck(b, ctx, xfer_fields(ctx, const_cast<des_fields *>(f), true, idx));
+}
+
+int des_batch_download_fields(des_batch *b, int model, des_fields *f)
+{
+    if (!b || !f) return DES_ERR_INPUT;
+    des_ctx *ctx;
+    int idx, rc = model_ctx(b, model, &ctx, &idx);
+    if (rc) return rc;
+    return bcheck(b, ctx, xfer_fields(ctx, f, false, idx));
+}
+
+int des_batch_set_scalars(des_batch *b, int model, double dt, double time, int steps,
+                          double compensation_pressure)
+{
+    if (!b) return DES_ERR_INPUT;
+    des_ctx *ctx;
+    int idx, rc = model_ctx(b, model, &ctx, &idx);
+    if (rc) return rc;
+    if ((rc = bcheck(b, ctx, flush_deferred(ctx)))) return rc;
+    DevScalars h{};
+    if ((rc = bcheck(b, ctx, read_model_scalars(ctx, idx, h)))) return rc;
+    h.dt = dt;
+    h.time = time;
+    h.steps = steps;
+    h.compensation_pressure = compensation_pressure;
+    h.rot_pending = 0;
+    hipError_t e = hipMemcpyAsync(ctx->s.scb + idx, &h, sizeof h, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return bfail(b, DES_ERR_HIP, hipGetErrorString(e));
+    b->msteps[model] = steps;
+    ctx->steps = steps;
+    return DES_OK;
+}
+
+int des_batch_get_scalars(des_batch *b, int model, double *dt, double *time, int *steps)
+{
+    if (!b) return DES_ERR_INPUT;
+    des_ctx *ctx;
+    int idx, rc = model_ctx(b, model, &ctx, &idx);
+    if (rc) return rc;
+    DevScalars h{};
+    if ((rc = bcheck(b, ctx, read_model_scalars(ctx, idx, h)))) return rc;
+    if (dt) *dt = h.dt;
+    if (time) *time = h.time;
+    if (steps) *steps = h.steps;
+    return DES_OK;
+}
+
+int des_batch_init_derived(des_batch *b)
+{
+    if (!b) return DES_ERR_INPUT;
+    for (des_ctx *c : b->groups) {
+        int rc = des_init_derived(c);
+        if (rc) return bcheck(b, c, rc);
+    }
+    return DES_OK;
+}
+
+int des_batch_apply_vbcs(des_batch *b)
+{
+    if (!b) return DES_ERR_INPUT;
+    for (des_ctx *c : b->groups) {
+        int rc = des_apply_vbcs(c);
+        if (rc) return bcheck(b, c, rc);
+    }
+    return DES_OK;
+}
+
+int des_batch_compute_dt(des_batch *b, double *dt)
+{
+    if (!b) return DES_ERR_INPUT;
+    for (des_ctx *ctx : b->groups) {
+        int rc = flush_deferred(ctx);
+        if (!rc) rc = launch_compute_dt(ctx);
+        if (rc) return bcheck(b, ctx, rc);
+    }
+    for (int m = 0; m < (int)b->group_of.size(); ++m) {
+        des_ctx *ctx = b->groups[b->group_of[m]];
+        DevScalars h{};
+        int rc = read_model_scalars(ctx, b->index_of[m], h);
+        if (rc) return bcheck(b, ctx, rc);
+        if (dt) dt[m] = h.dt;
+        if (h.err) return bfail(b, h.err, "compute_dt: dt <= 0 in model " + std::to_string(m));
+    }
+    return DES_OK;
+}
+
+int des_batch_run(des_batch *b, int nsteps, int precision)
+{
+    if (!b) return DES_ERR_INPUT;
+    if (precision != DES_FP64 && precision != DES_MIXED) return bfail(b, DES_ERR_INPUT, "unknown precision");
+    if (nsteps <= 0) return DES_OK;
+    for (size_t m = 1; m < b->msteps.size(); ++m)
+        if (b->msteps[m] != b->msteps[0])
+            return bfail(b, DES_ERR_STATE, "batched models must start a run at the same step count");
+    const bool mixed = precision == DES_MIXED;
+    std::vector<PassTimer> tms;
+    tms.reserve(b->groups.size());       // timers disabled: no events to own
+    for (des_ctx *c : b->groups) tms.emplace_back(c, false);
+    for (des_ctx *c : b->groups) {
+        int rc = fused_begin(c, precision);
+        if (rc) return bcheck(b, c, rc);
+    }
+    for (int k = 0; k < nsteps; ++k)
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            int rc = fused_step(b->groups[g], k, nsteps, mixed, tms[g]);
+            if (rc) return bcheck(b, b->groups[g], rc);
+        }
+    for (des_ctx *c : b->groups) {
+        int rc = fused_end(c, mixed);
+        if (rc) return bcheck(b, c, rc);
+    }
+    for (int &s : b->msteps) s += nsteps;
+    return DES_OK;
+}
+
+int des_batch_sync(des_batch *b)
+{
+    if (!b) return DES_ERR_INPUT;
+    for (des_ctx *ctx : b->groups) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return bfail(b, DES_ERR_HIP, hipGetErrorString(e));
+    }
+    for (des_ctx *ctx : b->groups) {
+        std::vector<DevScalars> h(ctx->nmodels);
+        hipError_t e = hipMemcpy(h.data(), ctx->s.scb, sizeof(DevScalars) * h.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return bfail(b, DES_ERR_HIP, hipGetErrorString(e));
+        for (int k = 0; k < ctx->nmodels; ++k)
+            if (h[k].err) {
+                for (size_t m = 0; m < b->group_of.size(); ++m)
+                    if (b->groups[b->group_of[m]] == ctx && b->index_of[m] == k)
+                        return bfail(b, h[k].err, "device reported an error in model " + std::to_string(m));
+            }
+    }
+    return DES_OK;
+}
+
+int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem)
+{
+    if (!b || !q || !worst_elem) return DES_ERR_INPUT;
+    for (des_ctx *ctx : b->groups) {
+        int rc = flush_deferred(ctx);
+        if (rc) return bcheck(b, ctx, rc);
+        const int neb = nblocks(ctx->E);
+        if (ctx->nd == 3) hipLaunchKernelGGL(k_quality_partial_b<3>, dim3(neb), dim3(BLOCK), 0, ctx->stream, ctx->s);
+        else hipLaunchKernelGGL(k_quality_partial_b<2>, dim3(neb), dim3(BLOCK), 0, ctx->stream, ctx->s);
+        if ((rc = check_launch(ctx))) return bcheck(b, ctx, rc);
+        std::vector<double> hq(neb);
+        std::vector<int> hi(neb);
+        hipError_t e = hipMemcpyAsync(hq.data(), ctx->s.red, sizeof(double) * neb, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hi.data(), ctx->s.redi, sizeof(int) * neb, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return bfail(b, DES_ERR_HIP, hipGetErrorString(e));
+        for (size_t m = 0; m < b->group_of.size(); ++m) {
+            if (b->groups[b->group_of[m]] != ctx) continue;
+            const int k = b->index_of[m];
+            // from q = 1 at element 0, the first strict minimum (geometry.cxx:458-512)
+            double best = 1;
+            int bi = 0;
+            const int b0 = ctx->e_off[k] / BLOCK, b1 = nblocks(ctx->e_off[k] + ctx->e_cnt[k]);
+            for (int blk = b0; blk < b1; ++blk)
+                if (hq[blk] < best) {
+                    best = hq[blk];
+                    bi = hi[blk] - ctx->e_off[k];
+                }
+            q[m] = best;
+            worst_elem[m] = bi;
+        }
+    }
+    return DES_OK;
+}
+
+int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4])
+{
+    if (!b || !variant) return DES_ERR_INPUT;
+    for (des_ctx *ctx : b->groups)
+        if (ctx->nd == ndims) {
+            *variant = ctx->passc_variant;
+            if (ms)
+                for (int v = 0; v < 4; ++v) ms[v] = ctx->passc_ms[v];
+            return DES_OK;
+        }
+    return bfail(b, DES_ERR_INPUT, "no group of that dimension");
+}
+
+int des_get_tuning(des_ctx *ctx, int *variant, double ms[4])
+{
+    if (!ctx || !variant) return DES_ERR_INPUT;
+    *variant = ctx->passc_variant;
+    if (ms)
+        for (int v = 0; v < 4; ++v) ms[v] = ctx->passc_ms[v];
     return DES_OK;
 }
 

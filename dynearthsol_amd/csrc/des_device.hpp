@@ -79,6 +79,11 @@ struct DevState {
     const int *eblk_model, *eblk_lim, *nblk_model, *nblk_lim;
     const des_params *Pb;                         // per-model parameters
     DevScalars *scb;                              // per-model scalars
+    const int *fmodel;                            // model of each traction-loaded facet slot
+    const double *bnb, *evb;                      // per-model boundary normals / edge vectors
+    const int *evvb;                              // per-model edge_valid
+    const int *mblk_off;                          // first element block of each model (+ total)
+    int nmodels;
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min

@@ -23,6 +23,11 @@ EXPORTS = (
     "des_apply_vbcs", "des_update_mesh", "des_rotate_stress",
     "des_compute_dt", "des_worst_quality", "des_step", "des_run", "des_run_timed",
     "des_sync", "des_bad_mesh_quality", "des_upload_elemmarkers", "des_write_frame",
+    "des_get_tuning", "des_batch_create", "des_batch_destroy", "des_batch_last_error",
+    "des_batch_upload_fields", "des_batch_download_fields", "des_batch_set_scalars",
+    "des_batch_get_scalars", "des_batch_init_derived", "des_batch_apply_vbcs",
+    "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
+    "des_batch_tuning",
 )
 
 PER_FUNCTION = (
@@ -71,6 +76,23 @@ def load():
     lib.des_bad_mesh_quality.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.des_upload_elemmarkers.argtypes = [vp, C.POINTER(C.c_int)]
     lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
+    lib.des_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
+    lib.des_batch_destroy.argtypes = [vp]
+    lib.des_batch_destroy.restype = None
+    lib.des_batch_last_error.argtypes = [vp]
+    lib.des_batch_last_error.restype = C.c_char_p
+    lib.des_batch_upload_fields.argtypes = [vp, C.c_int, C.POINTER(abi.DesFields)]
+    lib.des_batch_download_fields.argtypes = [vp, C.c_int, C.POINTER(abi.DesFields)]
+    lib.des_batch_set_scalars.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double]
+    lib.des_batch_get_scalars.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int)]
+    for f in ("init_derived", "apply_vbcs", "sync"):
+        getattr(lib, "des_batch_" + f).argtypes = [vp]
+    lib.des_batch_compute_dt.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.des_batch_run.argtypes = [vp, C.c_int, C.c_int]
+    lib.des_batch_worst_quality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.des_batch_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     _lib = lib
     return lib
 
@@ -117,14 +139,7 @@ class DesModel:
         self._check(getattr(self.lib, "des_" + name)(self.ctx))
 
     def upload(self, **arrays):
-        f = abi.DesFields()
-        keep = []
-        for k, a in arrays.items():
-            if a is None:
-                continue
-            a = np.ascontiguousarray(a, dtype=np.float64)
-            keep.append(a)
-            setattr(f, k, abi.dptr(a))
+        f, keep = _fields_struct(arrays)
         self._check(self.lib.des_upload_fields(self.ctx, C.byref(f)))
 
     def init(self):
@@ -187,25 +202,143 @@ class DesModel:
         self._check(self.lib.des_worst_quality(self.ctx, C.byref(q), C.byref(w)))
         return q.value, w.value
 
+    def tuning(self):
+        """(pass-C variant chosen by the autotuner or -1, [ms per variant])."""
+        v = C.c_int()
+        ms = (C.c_double * 4)()
+        self._check(self.lib.des_get_tuning(self.ctx, C.byref(v), ms))
+        return v.value, list(ms)
+
     def download(self, names=None):
         """Fields as numpy arrays in the reference's AoS layout."""
-        nd, npe, nstr, N, E = self.nd, self.npe, self.nstr, self.N, self.E
-        shapes = {
-            "coord": (N, nd), "coord0": (N, nd), "vel": (N, nd), "force": (N, nd),
-            "temperature": (N,), "mass": (N,), "tmass": (N,), "volume_n": (N,), "ntmp": (N,),
-            "volume": (E,), "volume_old": (E,), "edvoldt": (E,), "plstrain": (E,),
-            "delta_plstrain": (E,), "dpressure": (E,), "stressyy": (E,),
-            "strain_rate": (E, nstr), "strain": (E, nstr), "stress": (E, nstr),
-            "shpdx": (E, npe), "shpdz": (E, npe),
-        }
-        if nd == 3:
-            shapes["shpdy"] = (E, npe)
-        if names is None:
-            names = list(shapes)
-        out = {n: np.zeros(shapes[n]) for n in names}
-        f = abi.DesFields()
-        for n, a in out.items():
-            setattr(f, n, abi.dptr(a))
+        out, f = _field_buffers(self.model, names)
         self._check(self.lib.des_download_fields(self.ctx, C.byref(f)))
+        self.sync()
+        return out
+
+
+def _field_shapes(model):
+    nd, npe, nstr, N, E = model.ndims, model.npe, model.nstr, model.nnode, model.nelem
+    shapes = {
+        "coord": (N, nd), "coord0": (N, nd), "vel": (N, nd), "force": (N, nd),
+        "temperature": (N,), "mass": (N,), "tmass": (N,), "volume_n": (N,), "ntmp": (N,),
+        "volume": (E,), "volume_old": (E,), "edvoldt": (E,), "plstrain": (E,),
+        "delta_plstrain": (E,), "dpressure": (E,), "stressyy": (E,),
+        "strain_rate": (E, nstr), "strain": (E, nstr), "stress": (E, nstr),
+        "shpdx": (E, npe), "shpdz": (E, npe),
+    }
+    if nd == 3:
+        shapes["shpdy"] = (E, npe)
+    return shapes
+
+
+def _field_buffers(model, names):
+    shapes = _field_shapes(model)
+    if names is None:
+        names = list(shapes)
+    out = {n: np.zeros(shapes[n]) for n in names}
+    f = abi.DesFields()
+    for n, a in out.items():
+        setattr(f, n, abi.dptr(a))
+    return out, f
+
+
+def _fields_struct(arrays):
+    f = abi.DesFields()
+    keep = []
+    for k, a in arrays.items():
+        if a is None:
+            continue
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        setattr(f, k, abi.dptr(a))
+    return f, keep
+
+
+class DesBatch:
+    """Many dynearthsol_amd.model.Model instances stepping together in one
+    batched context per dimension (des_batch_* in include/des_amd.h); each
+    model is set up like DesModel.init."""
+
+    def __init__(self, models, init=True):
+        self.lib = load()
+        self.models = list(models)
+        n = len(self.models)
+        self._params = (abi.DesParams * n)(*[m.params for m in self.models])
+        self._mstructs = [m.mesh_struct() for m in self.models]
+        self._meshes = (abi.DesMesh * n)(*self._mstructs)
+        b = C.c_void_p()
+        rc = self.lib.des_batch_create(n, self._params, self._meshes, C.byref(b))
+        self.b = b
+        if rc:
+            msg = self.lib.des_batch_last_error(b).decode() if b.value else "des_batch_create failed"
+            self.close()
+            raise DesError(rc, msg)
+        if init:
+            self.init()
+
+    def close(self):
+        if self.b and self.b.value:
+            self.lib.des_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.models)
+
+    def _check(self, rc):
+        if rc:
+            raise DesError(rc, self.lib.des_batch_last_error(self.b).decode())
+
+    def upload(self, k, **arrays):
+        f, keep = _fields_struct(arrays)
+        self._check(self.lib.des_batch_upload_fields(self.b, k, C.byref(f)))
+
+    def init(self):
+        self._check(self.lib.des_batch_init_derived(self.b))
+        self._check(self.lib.des_batch_apply_vbcs(self.b))
+        for k, m in enumerate(self.models):
+            self.upload(k, temperature=m.temperature0, stress=m.stress0, strain=m.strain0,
+                        stressyy=m.stressyy0, plstrain=m.plstrain0, elem_visc=m.elem_visc,
+                        elem_cohesion=m.elem_cohesion)
+            self._check(self.lib.des_batch_set_scalars(self.b, k, 0.0, 0.0, 0, m.compensation_pressure))
+        return self.compute_dt()
+
+    def compute_dt(self):
+        dt = np.zeros(len(self.models))
+        self._check(self.lib.des_batch_compute_dt(self.b, abi.dptr(dt)))
+        return dt
+
+    def run(self, n, precision=abi.DES_FP64):
+        self._check(self.lib.des_batch_run(self.b, n, precision))
+
+    def sync(self):
+        self._check(self.lib.des_batch_sync(self.b))
+
+    def scalars(self, k):
+        dt, t, st = C.c_double(), C.c_double(), C.c_int()
+        self._check(self.lib.des_batch_get_scalars(self.b, k, C.byref(dt), C.byref(t), C.byref(st)))
+        return dt.value, t.value, st.value
+
+    def worst_quality(self):
+        q = np.zeros(len(self.models))
+        w = np.zeros(len(self.models), dtype=np.int32)
+        self._check(self.lib.des_batch_worst_quality(self.b, abi.dptr(q), abi.iptr(w)))
+        return q, w
+
+    def tuning(self, ndims):
+        v = C.c_int()
+        ms = (C.c_double * 4)()
+        self._check(self.lib.des_batch_tuning(self.b, ndims, C.byref(v), ms))
+        return v.value, list(ms)
+
+    def download(self, k, names=None):
+        out, f = _field_buffers(self.models[k], names)
+        self._check(self.lib.des_batch_download_fields(self.b, k, C.byref(f)))
         self.sync()
         return out

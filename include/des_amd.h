@@ -220,6 +220,38 @@ int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[8], int cou
    error the step detected (dt <= 0, missing edge vector). */
 int des_sync(des_ctx *ctx);
 
+/* The pass-C variant (minimum waves per SIMD 1..4 as 0..3) that the autotuner
+   of des_run chose, -1 before a run of 16 or more steps, and the milliseconds
+   it measured for each.  No counterpart in the reference. */
+int des_get_tuning(des_ctx *ctx, int *variant, double ms[4]);
+
+/* ---- batched models ----------------------------------------------------------
+
+   Many independent models stepping together, as a caller would run many
+   instances of the reference's main loop (dynearthsol.cxx:316-361) one after
+   another; the reference itself has no batch interface.  Models are numbered
+   0..nmodels-1 in the order given; each keeps its own parameters, mesh, fields
+   and scalars.  Models of one dimension must share nmat and
+   markers_per_element.  All models advance together: des_batch_run steps every
+   model nsteps times, with compute_dt every 10 steps per model. */
+typedef struct des_batch des_batch;
+
+int des_batch_create(int nmodels, const des_params *params, const des_mesh *meshes, des_batch **out);
+void des_batch_destroy(des_batch *b);
+const char *des_batch_last_error(const des_batch *b);
+int des_batch_upload_fields(des_batch *b, int model, const des_fields *f);
+int des_batch_download_fields(des_batch *b, int model, des_fields *f);
+int des_batch_set_scalars(des_batch *b, int model, double dt, double time, int steps,
+                          double compensation_pressure);
+int des_batch_get_scalars(des_batch *b, int model, double *dt, double *time, int *steps);
+int des_batch_init_derived(des_batch *b);            /* des_init_derived for every model */
+int des_batch_apply_vbcs(des_batch *b);              /* des_apply_vbcs for every model */
+int des_batch_compute_dt(des_batch *b, double *dt);  /* dt[model] */
+int des_batch_run(des_batch *b, int nsteps, int precision);
+int des_batch_sync(des_batch *b);
+int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem);   /* per model */
+int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4]);
+
 #ifdef __cplusplus
 }
 #endif
