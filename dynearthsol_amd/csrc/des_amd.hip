@@ -1051,6 +1051,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && s.ret_count) *s.ret_count = 0;   // pass C appends
     BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
@@ -1083,7 +1084,8 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
     s.ntmp[n] = dj / vn;
 }
 
-template <int ND> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl, int e)
+template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
+                                                                          int e)
 {
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
@@ -1131,8 +1133,15 @@ template <int ND> __device__ __forceinline__ void pass_c_body(const DevState &s,
     if (rot) jaumann<ND>(st, dtp, w);
     double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
     const double vol = s.volume[e];
-    bool ok = rheology_after_strain<ND>(s, e, dt, total_dv, elem_mean<ND>(s.T, c), vol, s.volume_old[e], st,
-                                        syy, edot, pl, dpl, dp);
+    bool ok = true, defer = false;
+    double trial[6], old_s = 0;
+    if (DEFER && ND == 3 && (s.P->rheol_type == DES_RH_EP || s.P->rheol_type == DES_RH_EVP)) {
+        defer = plastic_deferred3(s, e, dt, elem_mean<ND>(s.T, c), vol, s.volume_old[e], st, edot, pl, dpl,
+                                  dp, trial, old_s);
+    } else {
+        ok = rheology_after_strain<ND>(s, e, dt, total_dv, elem_mean<ND>(s.T, c), vol, s.volume_old[e], st,
+                                       syy, edot, pl, dpl, dp);
+    }
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     store_tensor<ND>(s.stress, E, e, st);
     if (s.P->is_plane_strain) s.stressyy[e] = syy;
@@ -1140,12 +1149,70 @@ template <int ND> __device__ __forceinline__ void pass_c_body(const DevState &s,
     s.delta_plstrain[e] = dpl;
     s.dpressure[e] = dp;
     s.escal[e] = dp * vol;          // NMD_stress's dp * volume (geometry.cxx:206)
+    if constexpr (DEFER) {
+        // append the deferred elements, one atomic per wavefront
+        const unsigned long long mask = __ballot(defer);
+        if (mask) {
+            const int lane = __lane_id();
+            const int leader = __ffsll((long long)mask) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(s.ret_count, __popcll(mask));
+            base = __shfl(base, leader);
+            if (defer) {
+                const int k = base + __popcll(mask & ((1ull << lane) - 1));
+                s.ret_elem[k] = e;
+                double *rv = s.ret_val + 8 * (long)k;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) rv[i] = trial[i];
+                rv[6] = old_s;
+            }
+        }
+    }
 }
 
 template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
     BIND_ELEM(s, e);
-    pass_c_body<ND>(s, fl, e);
+    pass_c_body<ND, true>(s, fl, e);
+}
+
+// Pass C's Mohr-Coulomb returns (rheology.cxx:151-299, 594-635) of the 3D
+// elements that pass C deferred, from their trial stresses; the Maxwell
+// stress that pass C stored stays when its J2 is the smaller one.
+__global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
+{
+    const int cnt = *s.ret_count;
+    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < cnt; k += gridDim.x * BLOCK) {
+        const int e = s.ret_elem[k];
+        DevState st = s;
+        if (s.eblk_model) bind_model<3>(st, s.eblk_model[e / BLOCK]);
+        const des_params *P = st.P;
+        const double *rv = s.ret_val + 8 * (long)k;
+        double sp[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sp[i] = rv[i];
+        const double old_s = rv[6];
+        const double bulkm = harmonic_mean(st, P->bulk_modulus, e);
+        const double shearm = harmonic_mean(st, P->shear_modulus, e);
+        const double pl = s.plstrain[e];
+        double amc, anphi, anpsi, hardn, ten_max, depls;
+        mat_plastic_props(st, e, pl, amc, anphi, anpsi, hardn, ten_max);
+        mc_return<3>(bulkm, shearm, amc, anphi, anpsi, hardn, ten_max, depls, sp);
+        bool take = true;
+        if (P->rheol_type == DES_RH_EVP) {
+            double sv[6];
+            load_tensor<3>(s.stress, s.E, e, sv);
+            take = !(second_invariant2<3>(sv) < second_invariant2<3>(sp));
+        }
+        if (take) {
+            store_tensor<3>(s.stress, s.E, e, sp);
+            s.plstrain[e] = pl + depls;
+            s.delta_plstrain[e] = depls;
+            const double dp = trace<3>(sp) - old_s;
+            s.dpressure[e] = dp;
+            s.escal[e] = dp * s.volume[e];
+        }
+    }
 }
 
 // pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
@@ -2266,6 +2333,18 @@ struct PassTimer {
 
 #define MARK(k) do { if ((rc = tm.mark(k))) return rc; } while (0)
 
+// the deferred-return list of pass C (3D), allocated on first use
+int ensure_retlist(des_ctx *ctx)
+{
+    DevState &s = ctx->s;
+    if (s.ret_count || ctx->nd != 3) return DES_OK;
+    int rc;
+    if ((rc = dalloc_n(ctx, &s.ret_elem, (size_t)ctx->E))) return rc;
+    if ((rc = dalloc_n(ctx, &s.ret_val, (size_t)ctx->E * 8))) return rc;
+    if ((rc = dalloc_n(ctx, &s.ret_count, 1))) return rc;
+    return DES_OK;
+}
+
 // device buffers of the mixed-precision state, allocated on first use
 int ensure_mixed(des_ctx *ctx)
 {
@@ -2316,6 +2395,9 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
         }
     }
 #undef PC
+    int rc = check_launch(ctx);
+    if (rc || ctx->nd != 3) return rc;
+    hipLaunchKernelGGL(k_pass_c_ret, dim3(std::min(nblocks(ctx->E), 2048)), dim3(BLOCK), 0, ctx->stream, ctx->s);
     return check_launch(ctx);
 }
 
@@ -2337,6 +2419,7 @@ int fused_begin(des_ctx *ctx, int precision)
     }
     int rc = flush_deferred(ctx);
     if (rc) return rc;
+    if ((rc = ensure_retlist(ctx))) return rc;
     if (precision == DES_MIXED) {
         if ((rc = ensure_mixed(ctx))) return rc;
         LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);

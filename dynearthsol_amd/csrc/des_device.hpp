@@ -84,6 +84,11 @@ struct DevState {
     const int *evvb;                              // per-model edge_valid
     const int *mblk_off;                          // first element block of each model (+ total)
     int nmodels;
+    // elements whose Mohr-Coulomb return is left to the return kernel of the
+    // fused schedule: count, element ids, and per entry the trial stress and
+    // the stress trace before the update (8 doubles)
+    int *ret_count, *ret_elem;
+    double *ret_val;
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
@@ -695,11 +700,30 @@ template <int ND> __device__ __forceinline__ void viscous(double bulkm, double v
 
 // Mohr-Coulomb return in principal space of an elastic trial stress s
 // (rheology.cxx:167-298)
+// Whether the 3D trial stress s is inside the Mohr-Coulomb surface with a
+// margin that no rounding of the eigen solve can close.  The eigenvalues of s
+// lie within m -+ sqrt(2/3) |s - m I|_F (m = mean stress), so
+// fs >= (m - r) - (m + r) anphi + amc and ft <= m + r - ten_max.  The margin,
+// 1e-7 of the stress scale, is far above the eigenvalue error of the
+// analytic solver (at worst about sqrt(eps) of the scale).  When this holds
+// the return leaves s unchanged, exactly as after the solve.
+__device__ __forceinline__ bool clearly_elastic3(const double *s, double amc, double anphi, double ten_max)
+{
+    const double m = (s[0] + s[1] + s[2]) / 3;
+    const double d0 = s[0] - m, d1 = s[1] - m, d2 = s[2] - m;
+    const double r = sqrt((2.0 / 3) * (d0 * d0 + d1 * d1 + d2 * d2 +
+                                       2 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+    const double tol = 1e-7 * (fabs(m) + r + fabs(amc) + fabs(ten_max)) * (1 + anphi);
+    return (m - r) - (m + r) * anphi + amc > tol && (m + r) - ten_max < -tol;
+}
+
 template <int ND> __device__ __forceinline__ void mc_return(double bulkm, double shearm, double amc,
                                                            double anphi, double anpsi, double hardn,
                                                            double ten_max, double &depls, double *s)
 {
     depls = 0;
+    if constexpr (ND == 3)
+        if (clearly_elastic3(s, amc, anphi, ten_max)) return;
     double p[ND];
     double v[3][3];
     double cos2t = 0, sin2t = 0;
@@ -941,6 +965,56 @@ template <int ND> __device__ __forceinline__ bool rheology_after_strain(const De
     }
     dpressure = trace<ND>(s) - old_s;
     return ok;
+}
+
+// The elasto-plastic / elasto-visco-plastic branch of update_stress
+// (rheology.cxx:573-635) for one 3D element, with the Mohr-Coulomb return
+// left to a second kernel when the elastic trial is not clearly elastic.
+// Returns true for such an element: s then holds the Maxwell stress (EVP) or
+// the trial stress (EP), trial and old_s what the return needs, and plstrain
+// and delta_plstrain are untouched.  Otherwise the element is complete with
+// the values rheology_after_strain gives it.
+__device__ __forceinline__ bool plastic_deferred3(const DevState &st, int e, double dt, double Tmean,
+                                                  double vol, double vol_old, double *s, const double *edot,
+                                                  double &plstrain, double &delta_plstrain,
+                                                  double &dpressure, double *trial, double &old_s)
+{
+    const des_params *P = st.P;
+    old_s = trace<3>(s);
+    double de[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) de[i] = edot[i] * dt;
+    const double bulkm = harmonic_mean(st, P->bulk_modulus, e);
+    const double shearm = harmonic_mean(st, P->shear_modulus, e);
+    const bool evp = P->rheol_type == DES_RH_EVP;
+    double sv[6], svII = 0;
+    if (evp) {
+        const double viscosity = mat_visc<3>(st, e, Tmean, edot);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sv[i] = s[i];
+        maxwell<3>(bulkm, shearm, viscosity, dt, vol / vol_old - 1, de, sv);
+        svII = second_invariant2<3>(sv);
+    }
+    double amc, anphi, anpsi, hardn, ten_max;
+    mat_plastic_props(st, e, plstrain, amc, anphi, anpsi, hardn, ten_max);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) trial[i] = s[i];
+    elastic<3>(bulkm, shearm, de, trial);
+    const bool defer = !clearly_elastic3(trial, amc, anphi, ten_max);
+    if (!defer && !(evp && svII < second_invariant2<3>(trial))) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = trial[i];
+        plstrain += 0.0;
+        delta_plstrain = 0;
+    } else if (evp) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = sv[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s[i] = trial[i];
+    }
+    dpressure = trace<3>(s) - old_s;
+    return defer;
 }
 
 // Body of update_stress for one element (rheology.cxx:520-641).  edot is

@@ -637,6 +637,34 @@ class Model:
             result += em[:, i] / p.bulk_modulus[i]
         return em.sum(axis=1) / result
 
+    def initial_dt(self):
+        """compute_dt (geometry.cxx:239-310) of the initial mesh, single
+        material, in numpy: the step size a fixed-dt run can be given."""
+        p = self.params
+        x = self.coord[self.conn]                      # E x NPE x ND
+        if self.ndims == 3:
+            e1, e2, e3 = x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]
+            vol = np.einsum("ij,ij->i", e1, np.cross(e2, e3)) / 6
+            faces = [(0, 1, 2), (0, 1, 3), (2, 3, 0), (2, 3, 1)]
+            area = np.max([np.linalg.norm(np.cross(x[:, b] - x[:, a], x[:, c] - x[:, a]), axis=1) / 2
+                           for a, b, c in faces], axis=0)
+            minh = 3 * vol / area
+        else:
+            ab, ac = x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]
+            vol = np.abs(ab[:, 0] * ac[:, 1] - ab[:, 1] * ac[:, 0]) / 2
+            maxl = np.sqrt(np.max([np.sum((x[:, j] - x[:, i]) ** 2, axis=1)
+                                   for i, j in [(0, 1), (1, 2), (0, 2)]], axis=0))
+            minh = 2 * vol / maxl
+        minl = float(np.min(minh))
+        dt_maxwell = 0.5 * p.visc_min / (1e-40 + p.shear_modulus[0])
+        dt_diffusion = 0.5 * minl * minl / p.therm_diff_max if p.has_thermal_diffusion else math.inf
+        dt_advection = 0.5 * minl / p.max_vbc_val
+        if p.is_quasi_static:
+            dt_elastic = 0.5 * minl / (p.max_vbc_val * p.inertial_scaling)
+        else:
+            dt_elastic = 0.5 * minl / math.sqrt(p.bulk_modulus[0] / p.rho0[0])
+        return min(dt_elastic, dt_maxwell, dt_advection, dt_diffusion) * p.dt_fraction
+
     def mesh_struct(self):
         """C-ABI mesh struct; keeps references so the arrays outlive it."""
         m = abi.DesMesh()
@@ -698,6 +726,39 @@ def config_ensemble(n2d=128, n3d=128, seed=6, size_scale=1.0):
                         "ic.weakzone_depth_min": 0.45, "ic.weakzone_depth_max": 0.55})
         models.append(Model(cfg, nd, nelem_target=ne, temperature="linear", mesh_seed=i))
     return models
+
+
+def config_adjoint(case="maxwell", nelem_target=200_000, log_visc=(22.0, 24.0)):
+    """The differentiable configuration: 50 x 50 x 10 km box, per-element
+    log10 viscosity U[22, 24] and cohesion U[20, 60] MPa (seed 3), visco-elastic
+    ("maxwell") or elasto-visco-plastic ("plastic"), constant density
+    (alpha = 0), 1 cm/yr extension, and a fixed dt equal to the initial
+    compute_dt."""
+    v = 1.0e-2 / YEAR2SEC
+    cfg = {
+        "mat.rheology_type": "maxwell" if case == "maxwell" else "elasto-visco-plastic",
+        "mat.rho0": [2800.0], "mat.alpha": [0.0], "mat.bulk_modulus": [50e9], "mat.shear_modulus": [30e9],
+        "mat.cohesion0": [44e6], "mat.cohesion1": [4e6],
+        "mat.friction_angle0": [30.0], "mat.friction_angle1": [5.0],
+        "mat.pls0": [0.0], "mat.pls1": [0.1],
+        "mat.min_viscosity": 1e18, "mat.max_viscosity": 1e26,
+        "mat.heat_capacity": [1000.0], "mat.therm_cond": [2.8],
+        "bc.surface_temperature": 273.0, "bc.mantle_temperature": 1573.0,
+        "bc.vbc_x0": 1, "bc.vbc_val_x0": -v, "bc.vbc_x1": 1, "bc.vbc_val_x1": v,
+        "bc.vbc_y0": 1, "bc.vbc_val_y0": 0.0, "bc.vbc_y1": 1, "bc.vbc_val_y1": 0.0,
+        "bc.vbc_z0": 1, "bc.vbc_val_z0": 0.0, "bc.vbc_z1": 0,
+        "bc.has_winkler_foundation": False, "bc.has_water_loading": False,
+        "control.is_quasi_static": True, "control.inertial_scaling": 1e5,
+        "control.damping_option": 1, "control.damping_factor": 0.8, "control.gravity": 10.0,
+        "mesh.xlength": 50e3, "mesh.ylength": 50e3, "mesh.zlength": 10e3, "mesh.resolution": 1e3,
+        "mesh.quality_check_step_interval": 1 << 30,
+        "ic.weakzone_option": 1, "ic.weakzone_halfwidth": 2.5,
+        "ic.weakzone_depth_min": 0.45, "ic.weakzone_depth_max": 0.55,
+    }
+    m = Model(cfg, 3, nelem_target=nelem_target, temperature="linear",
+              elem_visc=(log_visc[0], log_visc[1], 3), elem_cohesion=("uniform", 20e6, 60e6, 3))
+    m.params.fixed_dt = m.initial_dt()
+    return m
 
 
 def config_benchmark(which, nelem_target=None):

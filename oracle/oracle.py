@@ -121,6 +121,20 @@ REF_CALL = {
 }
 
 
+def adjoint_lib():
+    """The FP64 CPU reference gradient (des_adjoint.c)."""
+    lib = _lib("liboracle_adj.so")
+    lib.adj3_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _d, _d, _d]
+    lib.adj3_gradient.restype = C.c_int
+    lib.adj3_supported.argtypes = [C.c_void_p, C.c_void_p]
+    lib.adj3_supported.restype = C.c_int
+    lib.adj3_tet_grad.argtypes = [_d, _d]
+    lib.adj3_tet_grad_adj.argtypes = [_d, _d, _d]
+    lib.o3_sizeof_state.restype = C.c_int
+    assert lib.o3_sizeof_state() == C.sizeof(OState)
+    return lib
+
+
 class OracleState:
     """The reference's Variables as numpy arrays plus the C view of them.
 
@@ -235,6 +249,30 @@ class OracleState:
         w = C.c_int()
         q = getattr(self.lib, self.pf + "worst_elem_quality")(C.byref(self.S), C.byref(w))
         return q, w.value
+
+    def gradient(self, nsteps, coord_bar):
+        """nsteps steps, then the gradient of sum(coord_bar * final coord) with
+        respect to elem_visc and elem_cohesion (des_adjoint.c)."""
+        lib = adjoint_lib()
+        cb = np.ascontiguousarray(coord_bar, dtype=np.float64).reshape(-1)
+        assert cb.size == self.N * self.nd
+        vb = np.zeros(self.E)
+        hb = np.zeros(self.E)
+        rc = lib.adj3_gradient(C.byref(self.S), C.byref(self.params), nsteps, abi.dptr(cb),
+                               abi.dptr(vb), abi.dptr(hb))
+        self._sync_swap()
+        if rc:
+            raise ValueError("configuration not supported by the CPU adjoint")
+        return vb, hb
+
+    def set_elem_params(self, elem_visc=None, elem_cohesion=None):
+        """Replace the per-element viscosity / cohesion (own copies)."""
+        if elem_visc is not None:
+            self.elem_visc = np.ascontiguousarray(elem_visc, dtype=np.float64).copy()
+            self.S.elem_visc = abi.dptr(self.elem_visc)
+        if elem_cohesion is not None:
+            self.elem_cohesion = np.ascontiguousarray(elem_cohesion, dtype=np.float64).copy()
+            self.S.elem_cohesion = abi.dptr(self.elem_cohesion)
 
     def copy(self):
         """Deep copy of the state (same mesh)."""

@@ -34,7 +34,8 @@ def test_frame_roundtrip(name, tmp_path):
     # element diagnostics against the oracle's material functions at the same state
     o = OracleState(m, init=False)
     for k in o.f:
-        o.f[k][:] = got[k]
+        if k in got:
+            o.f[k][:] = got[k]
     rho_fn = getattr(o.lib, f"o{m.ndims}_mat_rho")
     visc_fn = getattr(o.lib, f"o{m.ndims}_mat_visc")
     idx = np.arange(0, m.nelem, 97)
