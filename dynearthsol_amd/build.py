@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdes_amd.so")
 
 SOURCES = ["des_amd.hip"]
-HEADERS = ["des_device.hpp"]
+HEADERS = ["des_device.hpp", "des_adjoint.inc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1451,6 +1451,14 @@ __global__ void k_step_begin(DevState s)
 
 // ================================================================ host side
 
+// a checkpoint of the differentiated run: the state a fused step reads at its start
+struct AdjCkpt {
+    std::vector<double *> a;      // device copies
+    DevScalars sc;
+    int vol_cur, steps;
+    bool closing_pending;
+};
+
 struct des_ctx {
     des_params P;
     int nd, npe, nstr, N, E, nbf;
@@ -1476,6 +1484,10 @@ struct des_ctx {
     std::vector<des_params> hP;
     std::vector<int> e_off, n_off, e_cnt, n_cnt, nbf_cnt;
     std::vector<char> has_visc, has_coh;  // per model: per-element viscosity / cohesion uploaded
+    // differentiated run (des_adj_forward / des_adj_backward)
+    std::vector<void *> adj_allocs;
+    std::vector<AdjCkpt> adj_ckpt;
+    int adj_nsteps = 0, adj_window = 0;
 };
 
 // host description of a batched group's concatenated mesh
@@ -1921,6 +1933,7 @@ void des_destroy(des_ctx *ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void *p : ctx->allocs) (void)hipFree(p);
+    for (void *p : ctx->adj_allocs) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -2918,3 +2931,5 @@ int des_get_tuning(des_ctx *ctx, int *variant, double ms[4])
 }
 
 }  // extern "C"
+
+#include "des_adjoint.inc"

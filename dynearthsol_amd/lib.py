@@ -27,7 +27,7 @@ EXPORTS = (
     "des_batch_upload_fields", "des_batch_download_fields", "des_batch_set_scalars",
     "des_batch_get_scalars", "des_batch_init_derived", "des_batch_apply_vbcs",
     "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
-    "des_batch_tuning",
+    "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release",
 )
 
 PER_FUNCTION = (
@@ -93,6 +93,10 @@ def load():
     lib.des_batch_run.argtypes = [vp, C.c_int, C.c_int]
     lib.des_batch_worst_quality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.des_batch_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.des_adj_forward.argtypes = [vp, C.c_int, C.c_int]
+    lib.des_adj_backward.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.des_adj_release.argtypes = [vp]
+    lib.des_adj_release.restype = None
     _lib = lib
     return lib
 
@@ -201,6 +205,37 @@ class DesModel:
         q, w = C.c_double(), C.c_int()
         self._check(self.lib.des_worst_quality(self.ctx, C.byref(q), C.byref(w)))
         return q.value, w.value
+
+    def reset(self):
+        """Back to the model's initial state (coordinates, fields, scalars),
+        keeping the context and its per-element parameters."""
+        m = self.model
+        z = np.zeros((self.N, self.nd))
+        self.upload(coord=m.coord, coord0=m.coord, vel=z, force=z,
+                    delta_plstrain=np.zeros(self.E), dpressure=np.zeros(self.E))
+        self.init()
+
+    def set_elem_params(self, elem_visc=None, elem_cohesion=None):
+        """Upload per-element viscosity and / or cohesion."""
+        self.upload(elem_visc=elem_visc, elem_cohesion=elem_cohesion)
+
+    def adj_forward(self, nsteps, window=100):
+        """nsteps steps of the differentiated run, checkpointing every window
+        steps (des_adj_forward)."""
+        self._check(self.lib.des_adj_forward(self.ctx, nsteps, window))
+
+    def adj_backward(self, coord_bar):
+        """Gradients of the loss whose gradient with respect to the final
+        coordinates is coord_bar (nnode x ndims): (d/d elem_visc, d/d elem_cohesion)."""
+        cb = np.ascontiguousarray(coord_bar, dtype=np.float64)
+        assert cb.shape == (self.N, self.nd)
+        vb = np.zeros(self.E)
+        hb = np.zeros(self.E)
+        self._check(self.lib.des_adj_backward(self.ctx, abi.dptr(cb), abi.dptr(vb), abi.dptr(hb)))
+        return vb, hb
+
+    def adj_release(self):
+        self.lib.des_adj_release(self.ctx)
 
     def tuning(self):
         """(pass-C variant chosen by the autotuner or -1, [ms per variant])."""

@@ -225,6 +225,30 @@ int des_sync(des_ctx *ctx);
    it measured for each.  No counterpart in the reference. */
 int des_get_tuning(des_ctx *ctx, int *variant, double ms[4]);
 
+/* ---- differentiated run (no counterpart in the reference) ------------------
+
+   Reverse-mode gradient of a loss of the final coordinates with respect to
+   the per-element viscosity (elem_visc) and cohesion (elem_cohesion), through
+   the fused step, for 3D models with one material, alpha = 0, fixed dt,
+   damping option 0 or 1, no boundary tractions and axis-aligned velocity
+   boundaries; rheology elastic, maxwell, elasto-plastic or
+   elasto-visco-plastic (maxwell and elasto-visco-plastic need elem_visc).
+   Other configurations return 1 (DES_ERR_INPUT). */
+
+/* nsteps fused FP64 steps from the current state, keeping a device checkpoint
+   every `window` steps; leaves the context at the final state. */
+int des_adj_forward(des_ctx *ctx, int nsteps, int window);
+
+/* coord_bar (nnode x ndims, AoS) = d loss / d final coordinates.  Recomputes
+   each window from its checkpoint and runs the reverse sweep; writes
+   visc_bar[e] = d loss / d elem_visc[e] (0 where the viscosity is clamped to
+   the min / max) and cohesion_bar[e] = d loss / d elem_cohesion[e] (either
+   may be null).  The context is left at the final state of the run. */
+int des_adj_backward(des_ctx *ctx, const double *coord_bar, double *visc_bar, double *cohesion_bar);
+
+/* Frees the checkpoints and the reverse sweep's buffers. */
+void des_adj_release(des_ctx *ctx);
+
 /* ---- batched models ----------------------------------------------------------
 
    Many independent models stepping together, as a caller would run many

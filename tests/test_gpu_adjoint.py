@@ -1,0 +1,120 @@
+"""The HIP adjoint (des_adj_forward / des_adj_backward) against the FP64 CPU
+reference gradient (oracle/des_adjoint.c, itself pinned by finite
+differences of the oracle), against finite differences of the HIP forward,
+and through the torch.autograd binding."""
+import math
+
+import numpy as np
+import pytest
+
+import scenarios as SC
+from dynearthsol_amd import model as M
+from oracle import OracleState
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(case, nelem=600, perturb=0.0, coh_scale=1.0, damping=None, log_visc=(19.0, 21.0)):
+    m = M.config_adjoint(case, nelem_target=nelem, log_visc=log_visc)
+    if damping is not None:
+        m.params.damping_option = damping
+    if perturb:
+        SC._perturb_stress(m, perturb, 11)
+        m.elem_cohesion[:] *= coh_scale
+    top = (m.bcflag & 32) != 0
+    cb = np.zeros((m.nnode, 3))
+    cb[top, 2] = np.random.default_rng(0).standard_normal(int(top.sum()))
+    return m, cb
+
+
+def _gpu_grad(m, cb, nsteps, window):
+    from dynearthsol_amd.lib import DesModel
+    g = DesModel(m)
+    g.adj_forward(nsteps, window)
+    vb, hb = g.adj_backward(cb)
+    return g, vb, hb
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("case,kw", [("maxwell", {}), ("maxwell", {"damping": 0}),
+                                     ("plastic", {"perturb": 30e6, "coh_scale": 0.2})])
+def test_gradient_matches_cpu_reference(case, kw):
+    """15 steps in windows of 4 (three full windows and a partial one)."""
+    m, cb = _case(case, **kw)
+    print(case, kw, "elements", m.nelem, flush=True)
+    g, vb, hb = _gpu_grad(m, cb, 15, 4)
+    o = OracleState(m)
+    vr, hr = o.gradient(15, cb)
+    ev = _rel(vb, vr)
+    eh = _rel(hb, hr) if np.any(hr) else float(np.max(np.abs(hb)))
+    print(f"{case}: viscosity gradient {ev:.2e}, cohesion gradient {eh:.2e}")
+    assert ev < 1e-8 and eh < 1e-8
+    # the forward leaves the final state: same as the oracle's
+    got = g.download(["coord"])["coord"]
+    assert SC.rel_err(got, o.f["coord"]) < 1e-12
+
+
+def test_gradient_matches_finite_differences():
+    """Central differences in log space of the HIP forward, on the elements
+    with the largest gradients: 1e-5 relative (maxwell, no damping)."""
+    from dynearthsol_amd.lib import DesModel
+    m, cb = _case("maxwell", nelem=3000, damping=0)
+    g, vb, _ = _gpu_grad(m, cb, 20, 8)
+    h = 1e-3
+    for e in np.argsort(-np.abs(vb * m.elem_visc))[:6]:
+        vals = []
+        for sgn in (1, -1):
+            a = m.elem_visc.copy()
+            a[e] *= math.exp(sgn * h)
+            gg = DesModel(m, init=False)
+            gg.set_elem_params(elem_visc=a)
+            gg.init()
+            gg.run(20)
+            vals.append(float(np.sum(cb * gg.download(["coord"])["coord"])))
+        fd = (vals[0] - vals[1]) / (2 * h) / m.elem_visc[e]
+        print(f"e {e}: adjoint {vb[e]:.8e} fd {fd:.8e}", flush=True)
+        assert abs(vb[e] - fd) <= 1e-5 * abs(fd)
+
+
+def test_window_size_does_not_change_the_gradient():
+    """Recomputation from checkpoints reproduces the forward exactly: the
+    gradient is bitwise the same for any window."""
+    m, cb = _case("plastic", perturb=30e6, coh_scale=0.2)
+    _, v1, h1 = _gpu_grad(m, cb, 12, 3)
+    _, v2, h2 = _gpu_grad(m, cb, 12, 12)
+    _, v3, h3 = _gpu_grad(m, cb, 12, 5)
+    assert np.array_equal(v1, v2) and np.array_equal(h1, h2)
+    assert np.array_equal(v1, v3) and np.array_equal(h1, h3)
+
+
+def test_torch_autograd_binding():
+    import torch
+    from dynearthsol_amd.autograd import SurfaceTopography
+    m, _ = _case("maxwell")
+    topo = SurfaceTopography(m, nsteps=10, window=4)
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    lv = torch.tensor(np.log10(m.elem_visc), dtype=torch.float64, device=dev, requires_grad=True)
+    coh = torch.tensor(m.elem_cohesion, dtype=torch.float64, device=dev, requires_grad=True)
+    uz = topo(lv, coh)
+    target = uz.detach() * 0.9
+    loss = ((uz - target) ** 2).sum()
+    loss.backward()
+    # the same gradient straight from the C-ABI
+    cb = np.zeros((m.nnode, 3))
+    cb[topo.top, 2] = 2 * (uz - target).detach().cpu().numpy()
+    _, vb, hb = _gpu_grad(m, cb, 10, 4)
+    assert np.allclose(lv.grad.cpu().numpy(), vb * m.elem_visc * math.log(10), rtol=1e-12, atol=0)
+    assert np.array_equal(coh.grad.cpu().numpy(), hb)
+    # a second evaluation restarts from the initial state
+    uz2 = topo(lv.detach(), coh.detach())
+    assert torch.equal(uz2, uz.detach())
+
+
+def test_unsupported_configuration_is_refused():
+    from dynearthsol_amd.lib import DesError, DesModel
+    g = DesModel(SC.build("evp3"))
+    with pytest.raises(DesError):
+        g.adj_forward(2, 1)
