@@ -127,6 +127,62 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+class Profiler:
+    """roctxProfilerPause / Resume from ROCm's ROCTx library: a profiler that
+    honours them (rocprofv3) records only the primary measurement; without a
+    profiler the calls do nothing."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+
+def adjoint_timing(nsteps, window, nelem):
+    """The differentiated run of the visco-elastic adjoint configuration:
+    plain forward, checkpointed forward and reverse sweep, ms per step."""
+    from dynearthsol_amd import model as M
+    from dynearthsol_amd.lib import DesModel
+    m = M.config_adjoint("maxwell", nelem_target=nelem)
+    g = DesModel(m)
+    t_plain = timed(g, nsteps, 0)
+    g.reset()
+    g.sync()
+    t0 = time.perf_counter()
+    g.adj_forward(nsteps, window)
+    g.sync()
+    t_fwd = time.perf_counter() - t0
+    top = (m.bcflag & 32) != 0
+    cb = np.zeros((m.nnode, 3))
+    cb[top, 2] = 1.0
+    t0 = time.perf_counter()
+    g.adj_backward(cb)
+    t_bwd = time.perf_counter() - t0
+    g.close()
+    return {"workload": f"adjoint box 50x50x10 km, {m.nelem} tets, maxwell, log10 viscosity U[22,24]",
+            "steps": nsteps, "window": window,
+            "forward_ms_per_step": 1e3 * t_plain / nsteps,
+            "checkpointed_forward_ms_per_step": 1e3 * t_fwd / nsteps,
+            "backward_ms_per_step": 1e3 * t_bwd / nsteps,
+            "gradient_over_forward": (t_fwd + t_bwd) / t_plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +193,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--ensemble-scale", type=float, default=1.0)
     ap.add_argument("--seq-steps", type=int, default=100)
     args = ap.parse_args()
@@ -150,22 +208,31 @@ def main():
     from dynearthsol_amd import abi, model as M
     from dynearthsol_amd.lib import DesModel
 
+    prof = Profiler()
+    prof.pause()
     desc, ne_default = WORKLOADS[args.workload]
     nelem = args.nelem or ne_default
     primary = abi.DES_MIXED if args.workload == "3d-mixed" else abi.DES_FP64
     m = M.config_benchmark(args.workload, nelem_target=nelem)
     E, N = m.nelem, m.nnode
-    g = DesModel(m)
+    # two identical models: g runs the timed steps, its twin the same steps
+    # under per-pass HIP events (the step is deterministic, so both see the
+    # same states); the profiler records exactly these two runs
+    g, twin = DesModel(m), DesModel(m)
     print(f"# {args.workload}: {E} elements, {N} nodes; warmup {args.warmup}", file=sys.stderr, flush=True)
     g.run(args.warmup, primary)
+    twin.set_tuning(g.tuning()[0])
+    twin.run(args.warmup, primary)
+    g.sync()
+    twin.sync()
+    prof.resume()
     secs = timed(g, args.steps, primary)
+    passes = twin.run_timed(args.steps, primary)
+    twin.sync()
+    prof.pause()
     ms_per_step = 1e3 * secs / args.steps
     value = E * args.steps / secs
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
-
-    # per-pass durations with HIP events on the library's stream, same number of steps
-    passes = g.run_timed(args.steps, primary)
-    g.sync()
     avg = {k: ms / max(c, 1) for k, (ms, c) in passes.items()}
     table = PASS_BYTES_3D_MIXED if primary == abi.DES_MIXED else PASS_BYTES_3D
     dom = max((k for k in avg if k in table), key=lambda k: avg[k])
@@ -173,25 +240,31 @@ def main():
     achieved = (be * E + bn * N) / (avg[dom] * 1e-3) / 1e9
     step_bytes = sum(be_ * E + bn_ * N for be_, bn_ in table.values())
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+    kernel = {"pass_c": "k_pass_c" if primary == abi.DES_FP64 else "k_pass_c_mixed"}.get(dom, "k_" + dom)
 
-    # FP64 and mixed precision side by side from the same initial state
+    # the other precision from the same initial state, over the same steps
     compare = None
     if not args.no_mixed:
         other = abi.DES_FP64 if primary == abi.DES_MIXED else abi.DES_MIXED
-        g1, g2 = DesModel(m), DesModel(m)
-        n_cmp = min(args.steps, 2000)
-        g1.run(args.warmup, abi.DES_FP64)
-        g2.run(args.warmup, abi.DES_MIXED)
-        t64 = timed(g1, n_cmp, abi.DES_FP64)
-        t32 = timed(g2, n_cmp, abi.DES_MIXED)
-        a, b = g2.download(["vel", "stress", "coord", "plstrain"]), g1.download(["vel", "stress", "coord", "plstrain"])
-        compare = {"steps_after_warmup": n_cmp + args.warmup,
-                   "fp64_ms_per_step": 1e3 * t64 / n_cmp, "mixed_ms_per_step": 1e3 * t32 / n_cmp,
+        g2 = DesModel(m)
+        g2.run(args.warmup, other)
+        t2 = timed(g2, args.steps, other)
+        names = ["vel", "stress", "coord", "plstrain"]
+        a, b = g2.download(names), g.download(names)
+        t64, t32 = (secs, t2) if primary == abi.DES_FP64 else (t2, secs)
+        if primary != abi.DES_FP64:
+            a, b = b, a
+        compare = {"steps": args.warmup + args.steps,
+                   "fp64_ms_per_step": 1e3 * t64 / args.steps, "mixed_ms_per_step": 1e3 * t32 / args.steps,
                    "mixed_speedup": t64 / t32,
-                   "mixed_rel_err_vs_fp64": {k: rel(a[k], b[k]) for k in a}}
-        g1.close()
+                   "mixed_rel_err_vs_fp64": {k: rel(a[k], b[k]) for k in names}}
         g2.close()
-        del other
+    g.close()
+    twin.close()
+
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_timing(args.adjoint_steps, 100, 200_000)
 
     cpu = None
     if not args.no_cpu:
@@ -212,12 +285,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
                    "precision": "fp64" if primary == abi.DES_FP64 else "mixed"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kernel, "pass": dom, "avg_ms": avg[dom],
+                     "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "step_hbm": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},
         "fp64_vs_mixed": compare,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -30,8 +30,8 @@ class _Run(torch.autograd.Function):
         g = topo.g
         visc = 10.0 ** log_visc.detach().double().cpu().numpy()
         coh = cohesion.detach().double().cpu().numpy()
-        g.set_elem_params(elem_visc=visc, elem_cohesion=coh)
         g.reset()
+        g.set_elem_params(elem_visc=visc, elem_cohesion=coh)
         g.adj_forward(topo.nsteps, topo.window)
         z = g.download(["coord"])["coord"][topo.top, -1]
         uz = z - topo.z0

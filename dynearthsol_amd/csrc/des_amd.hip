@@ -507,10 +507,28 @@ template <int ND> __device__ __forceinline__ void node_force(const DevState &s, 
     const des_params *P = s.P;
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[d] = 0;
-    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) {
-        const double *c = s.ework + (long)s.csr_ent[p] * ND;
+    {
+        const int p1 = s.csr_off[n + 1];
+        int p = s.csr_off[n];
+        for (; p + 4 <= p1; p += 4) {   // four entries in flight, summed in order
+            int en[4];
+            double c[4][ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) f[d] -= c[d];
+            for (int j = 0; j < 4; ++j) en[j] = s.csr_ent[p + j];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) c[j][d] = s.ework[(long)en[j] * ND + d];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) f[d] -= c[j][d];
+        }
+        for (; p < p1; ++p) {
+            const double *c = s.ework + (long)s.csr_ent[p] * ND;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) f[d] -= c[d];
+        }
     }
     if (s.ff_off) {
         for (int p = s.ff_off[n]; p < s.ff_off[n + 1]; ++p) {
@@ -1020,19 +1038,15 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, 
     const des_params *P = s.P;
     int c[NPE];
     load_conn<ND>(s, e, c);
-    double x[NPE][ND], sh[ND][NPE], v[NPE][ND];
-    gather_vec<ND>(s.coord, s.N, c, x);
+    double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
+    load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
     double vol = elem_volume<ND>(x);
     shape_fn<ND>(x, vol, sh);
     s.volume[e] = vol;
     double rec[REC];
 #pragma unroll
     for (int k = 0; k < REC; ++k) rec[k] = 0;
-    double Tn[NPE];
-#pragma unroll
-    for (int i = 0; i < NPE; ++i) Tn[i] = s.T[c[i]];
     if (P->has_thermal_diffusion) temperature_contrib<ND>(s, e, vol, sh, Tn, rec);
-    gather_vec<ND>(s.vel, s.N, c, v);
     double sr[NSTR];
     strain_rate<ND>(v, sh, sr);
     rec[REC_DJ] = trace<ND>(sr) * vol;
@@ -1057,19 +1071,42 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
     const int thermal = s.P->has_thermal_diffusion;
     double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
     const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
-    for (int p = p0; p < p1; ++p) {
-        const int ent = s.csr_ent[p];
-        const int e = ent / NPE, i = ent - e * NPE;
-        const double *r = s.erec + (long)e * REC;
-        const double2 djm = *reinterpret_cast<const double2 *>(r + REC_DJ);
-        const double2 tmv = *reinterpret_cast<const double2 *>(r + REC_TM);
-        if (thermal) tdot += r[i];
+    // CSR gather in ascending element order, four entries in flight at a time
+    auto entry = [&](int ent, double ti, double2 djm, double2 tmv) {
+        (void)ent;
+        if (thermal) tdot += ti;
         dj += djm.x;
         if (!fl.use_stored_mass) {
             vn += tmv.y;
             m += djm.y;
             if (thermal) tm += tmv.x;
         }
+    };
+    auto load = [&](int ent, double &ti, double2 &djm, double2 &tmv) {
+        const int e = ent / NPE, i = ent - e * NPE;
+        const double *r = s.erec + (long)e * REC;
+        djm = *reinterpret_cast<const double2 *>(r + REC_DJ);
+        tmv = *reinterpret_cast<const double2 *>(r + REC_TM);
+        ti = thermal ? r[i] : 0.0;
+    };
+    int p = p0;
+    for (; p + 4 <= p1; p += 4) {
+        int en[4];
+        double ti[4];
+        double2 djm[4], tmv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) en[j] = s.csr_ent[p + j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) load(en[j], ti[j], djm[j], tmv[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) entry(en[j], ti[j], djm[j], tmv[j]);
+    }
+    for (; p < p1; ++p) {
+        double ti;
+        double2 djm, tmv;
+        const int en = s.csr_ent[p];
+        load(en, ti, djm, tmv);
+        entry(en, ti, djm, tmv);
     }
     if (fl.use_stored_mass) {
         vn = s.volume_n[n];
@@ -1080,8 +1117,15 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
         s.mass[n] = m;
         s.tmass[n] = tm;
     }
-    if (thermal) s.T[n] = temperature_new(s, n, s.T[n], tdot, s.sc->dt, tm);
-    s.ntmp[n] = dj / vn;
+    double *rn = s.nrec + (long)NREC * n;
+    if (thermal) {
+        const double Tn = temperature_new(s, n, s.T[n], tdot, s.sc->dt, tm);
+        s.T[n] = Tn;
+        rn[NrecSlot<ND>::T] = Tn;
+    }
+    const double dvn = dj / vn;
+    s.ntmp[n] = dvn;
+    rn[NrecSlot<ND>::AUX] = dvn;
 }
 
 template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
@@ -1091,13 +1135,12 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     const long E = s.E;
     int c[NPE];
     load_conn<ND>(s, e, c);
-    double sh[ND][NPE], v[NPE][ND];
+    double sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
     {
         double x[NPE][ND];
-        gather_vec<ND>(s.coord, s.N, c, x);
+        load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
         shape_fn<ND>(x, s.volume[e], sh);
     }
-    gather_vec<ND>(s.vel, s.N, c, v);
     double w[3] = {0, 0, 0};
     const bool rot = s.sc->rot_pending;
     if (rot) spin<ND>(v, sh, w);
@@ -1106,7 +1149,7 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     strain_rate<ND>(v, sh, edot);
     double edv = 0;
 #pragma unroll
-    for (int i = 0; i < NPE; ++i) edv += s.ntmp[c[i]];
+    for (int i = 0; i < NPE; ++i) edv += aux[i];
     edv = edv / NPE;
     const double dt = s.sc->dt;
     // anti-locking and the strain update of update_stress (rheology.cxx:528-540)
@@ -1135,12 +1178,12 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     const double vol = s.volume[e];
     bool ok = true, defer = false;
     double trial[6], old_s = 0;
+    const double Tmean = node_mean<ND>(Tn);
     if (DEFER && ND == 3 && (s.P->rheol_type == DES_RH_EP || s.P->rheol_type == DES_RH_EVP)) {
-        defer = plastic_deferred3(s, e, dt, elem_mean<ND>(s.T, c), vol, s.volume_old[e], st, edot, pl, dpl,
-                                  dp, trial, old_s);
+        defer = plastic_deferred3(s, e, dt, Tmean, vol, s.volume_old[e], st, edot, pl, dpl, dp, trial, old_s);
     } else {
-        ok = rheology_after_strain<ND>(s, e, dt, total_dv, elem_mean<ND>(s.T, c), vol, s.volume_old[e], st,
-                                       syy, edot, pl, dpl, dp);
+        ok = rheology_after_strain<ND>(s, e, dt, total_dv, Tmean, vol, s.volume_old[e], st, syy, edot, pl, dpl,
+                                       dp);
     }
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     store_tensor<ND>(s.stress, E, e, st);
@@ -1221,8 +1264,22 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s)
     BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
-    for (int p = s.csr_off[n]; p < s.csr_off[n + 1]; ++p) sum += s.escal[s.csr_ent[p] / NPE];
-    s.ntmp[n] = sum / s.volume_n[n];
+    const int p1 = s.csr_off[n + 1];
+    int p = s.csr_off[n];
+    for (; p + 4 <= p1; p += 4) {       // four entries in flight, summed in order
+        int en[4];
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) en[j] = s.csr_ent[p + j];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = s.escal[en[j] / NPE];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sum += v[j];
+    }
+    for (; p < p1; ++p) sum += s.escal[s.csr_ent[p] / NPE];
+    const double dpn = sum / s.volume_n[n];
+    s.ntmp[n] = dpn;
+    s.nrec[(long)NREC * n + NrecSlot<ND>::AUX] = dpn;
 }
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
@@ -1232,16 +1289,22 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, 
     const long E = s.E;
     int c[NPE];
     load_conn<ND>(s, e, c);
+    double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
+    load_nrec<ND>(s.nrec, c, x, v, Tn, dpn);
     double st[NSTR];
     load_tensor<ND>(s.stress, E, e, st);
-    nmd_correct<ND>(s, e, c, st);
+    {   // NMD_stress's element side (geometry.cxx:221-235), dp_nd from the records
+        const double dp_el = node_mean<ND>(dpn);
+        const double dp_orig = s.dpressure[e];
+#pragma unroll
+        for (int i = 0; i < ND; ++i) st[i] += (-dp_orig + dp_el) / ND;
+    }
 #pragma unroll
     for (int i = 0; i < ND; ++i) s.stress[i * E + e] = st[i];
-    double x[NPE][ND], sh[ND][NPE], out[NPE][ND];
-    gather_vec<ND>(s.coord, s.N, c, x);
+    double sh[ND][NPE], out[NPE][ND];
     const double vol = s.volume[e];
     shape_fn<ND>(x, vol, sh);
-    force_contrib<ND>(s, e, vol, sh, st, elem_mean<ND>(s.T, c), out);
+    force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     store_force_contrib<ND>(s, e, out);
 }
 
@@ -1263,10 +1326,14 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
     unsigned flag = s.bcflag[n];
     if (flag & BOUND_ANY)
         if (!vbc_node<ND>(s, flag, v)) set_error(s.sc, DES_ERR_ASSERT);
+    double *rn = s.nrec + (long)NREC * n;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         s.vel[d * s.N + n] = v[d];
-        s.coord[d * s.N + n] += v[d] * dt;
+        const double xn = s.coord[d * s.N + n] + v[d] * dt;
+        s.coord[d * s.N + n] = xn;
+        rn[NrecSlot<ND>::X + d] = xn;
+        rn[NrecSlot<ND>::V + d] = v[d];
     }
 }
 
@@ -1317,11 +1384,10 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     const long E = s.E;
     int c[NPE];
     load_conn<ND>(s, e, c);
-    double x[NPE][ND], sh[ND][NPE], v[NPE][ND];
-    gather_vec<ND>(s.coord, s.N, c, x);
+    double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
+    load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
     const double vol = s.volume[e];
     shape_fn<ND>(x, vol, sh);
-    gather_vec<ND>(s.vel, s.N, c, v);
     // velocities relative to node 0 (sum_i dN_i = 0): no cancellation of the
     // rigid-body part when the rates are rounded to FP32
 #pragma unroll
@@ -1354,11 +1420,11 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     }
     double edv = 0;
 #pragma unroll
-    for (int i = 0; i < NPE; ++i) edv += s.ntmp[c[i]];
+    for (int i = 0; i < NPE; ++i) edv += aux[i];
     edv = edv / NPE;
     float pl = s.pl32[e], dpl = s.dpl32[e];
     double dp;
-    const float Tm = (float)elem_mean<ND>(s.T, c);
+    const float Tm = (float)node_mean<ND>(Tn);
     bool ok = stress_update_mixed<ND>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p, sd,
                                       rs, es, re, edot, pl, dpl, dp);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
@@ -1386,10 +1452,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevSta
     const long E = s.E;
     int c[NPE];
     load_conn<ND>(s, e, c);
-    double dpn = 0;
-#pragma unroll
-    for (int i = 0; i < NPE; ++i) dpn += s.ntmp[c[i]];
-    const double dp_el = dpn / NPE;
+    double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
+    load_nrec<ND>(s.nrec, c, x, v, Tn, dpn);
+    const double dp_el = node_mean<ND>(dpn);
     double p = s.pres[e] + (-s.dpressure[e] + dp_el) / ND;   // NMD_stress on the diagonal mean
     s.pres[e] = p;
     float sd[NSTR];
@@ -1397,12 +1462,28 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevSta
     for (int k = 0; k < NSTR; ++k) sd[k] = s.sdev[k * E + e];
     double st[NSTR];
     join_stress<ND>(p, sd, st);
-    double x[NPE][ND], sh[ND][NPE], out[NPE][ND];
-    gather_vec<ND>(s.coord, s.N, c, x);
+    double sh[ND][NPE], out[NPE][ND];
     const double vol = s.volume[e];
     shape_fn<ND>(x, vol, sh);
-    force_contrib<ND>(s, e, vol, sh, st, elem_mean<ND>(s.T, c), out);
+    force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     store_force_contrib<ND>(s, e, out);
+}
+
+// node records from the SoA state, at the start of a fused run
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pack_nrec(DevState s)
+{
+    const int n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= s.N) return;
+    double *rn = s.nrec + (long)NREC * n;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        rn[NrecSlot<ND>::X + d] = s.coord[d * s.N + n];
+        rn[NrecSlot<ND>::V + d] = s.vel[d * s.N + n];
+    }
+    rn[NrecSlot<ND>::T] = s.T[n];
+    rn[NrecSlot<ND>::AUX] = s.ntmp[n];
+#pragma unroll
+    for (int k = 2 * ND + 2; k < NREC; ++k) rn[k] = 0;
 }
 
 // step bookkeeping after pass F: var.time, var.steps, the rotation to defer
@@ -2350,6 +2431,10 @@ struct PassTimer {
 int ensure_retlist(des_ctx *ctx)
 {
     DevState &s = ctx->s;
+    if (!s.nrec) {
+        int rc = dalloc_n(ctx, &s.nrec, (size_t)ctx->N * NREC);
+        if (rc) return rc;
+    }
     if (s.ret_count || ctx->nd != 3) return DES_OK;
     int rc;
     if ((rc = dalloc_n(ctx, &s.ret_elem, (size_t)ctx->E))) return rc;
@@ -2408,8 +2493,13 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
         }
     }
 #undef PC
-    int rc = check_launch(ctx);
-    if (rc || ctx->nd != 3) return rc;
+    return check_launch(ctx);
+}
+
+// the Mohr-Coulomb returns that pass C deferred (3D)
+int launch_pass_c_ret(des_ctx *ctx)
+{
+    if (ctx->nd != 3) return DES_OK;
     hipLaunchKernelGGL(k_pass_c_ret, dim3(std::min(nblocks(ctx->E), 2048)), dim3(BLOCK), 0, ctx->stream, ctx->s);
     return check_launch(ctx);
 }
@@ -2433,6 +2523,7 @@ int fused_begin(des_ctx *ctx, int precision)
     int rc = flush_deferred(ctx);
     if (rc) return rc;
     if ((rc = ensure_retlist(ctx))) return rc;
+    LAUNCH_ND(k_pack_nrec, nblocks(ctx->N), ctx->s);
     if (precision == DES_MIXED) {
         if ((rc = ensure_mixed(ctx))) return rc;
         LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);
@@ -2488,6 +2579,10 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
     }
     MARK(2);
+    if (!mixed) {
+        if ((rc = launch_pass_c_ret(ctx))) return rc;
+        MARK(8);
+    }
     LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s);
     MARK(3);
     if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
@@ -2543,6 +2638,13 @@ int des_run(des_ctx *ctx, int nsteps, int precision)
     if (!ctx) return DES_ERR_INPUT;
     PassTimer tm(ctx, false);
     return run_fused(ctx, nsteps, precision, tm);
+}
+
+int des_set_tuning(des_ctx *ctx, int variant)
+{
+    if (!ctx || variant < -1 || variant >= PASSC_VARIANTS) return DES_ERR_INPUT;
+    ctx->passc_variant = variant;
+    return DES_OK;
 }
 
 int des_run_timed(des_ctx *ctx, int nsteps, int precision, double *ms, int *counts)

@@ -89,7 +89,50 @@ struct DevState {
     // the stress trace before the update (8 doubles)
     int *ret_count, *ret_elem;
     double *ret_val;
+    // node records of the fused schedule (NREC doubles per node): x[ND],
+    // v[ND], T, aux (dvoldt after pass B, dp_nd after pass D), one cache
+    // line per node for the element passes' gathers
+    double *nrec;
 };
+
+constexpr int NREC = 8;
+
+// slots of the node record
+template <int ND> struct NrecSlot {
+    static constexpr int X = 0, V = ND, T = 2 * ND, AUX = 2 * ND + 1;
+};
+
+// the records of an element's nodes: coordinates, velocities, T and aux
+template <int ND> __device__ __forceinline__ void load_nrec(const double *nrec, const int *c,
+                                                          double x[Dim<ND>::NPE][ND],
+                                                          double v[Dim<ND>::NPE][ND],
+                                                          double T[Dim<ND>::NPE], double aux[Dim<ND>::NPE])
+{
+#pragma unroll
+    for (int i = 0; i < Dim<ND>::NPE; ++i) {
+        const double2 *r = reinterpret_cast<const double2 *>(nrec + (long)NREC * c[i]);
+        if constexpr (ND == 3) {
+            const double2 a = r[0], b = r[1], d = r[2], f = r[3];
+            x[i][0] = a.x; x[i][1] = a.y; x[i][2] = b.x;
+            v[i][0] = b.y; v[i][1] = d.x; v[i][2] = d.y;
+            T[i] = f.x; aux[i] = f.y;
+        } else {
+            const double2 a = r[0], b = r[1], d = r[2];
+            x[i][0] = a.x; x[i][1] = a.y;
+            v[i][0] = b.x; v[i][1] = b.y;
+            T[i] = d.x; aux[i] = d.y;
+        }
+    }
+}
+
+// mean over an element's nodes, in node order (elem_mean's summation)
+template <int ND> __device__ __forceinline__ double node_mean(const double a[Dim<ND>::NPE])
+{
+    double t = 0;
+#pragma unroll
+    for (int i = 0; i < Dim<ND>::NPE; ++i) t += a[i];
+    return t / Dim<ND>::NPE;
+}
 
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max

@@ -27,7 +27,7 @@ EXPORTS = (
     "des_batch_upload_fields", "des_batch_download_fields", "des_batch_set_scalars",
     "des_batch_get_scalars", "des_batch_init_derived", "des_batch_apply_vbcs",
     "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
-    "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release",
+    "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
 )
 
 PER_FUNCTION = (
@@ -77,6 +77,7 @@ def load():
     lib.des_upload_elemmarkers.argtypes = [vp, C.POINTER(C.c_int)]
     lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     lib.des_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.des_set_tuning.argtypes = [vp, C.c_int]
     lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
     lib.des_batch_destroy.argtypes = [vp]
     lib.des_batch_destroy.restype = None
@@ -168,14 +169,17 @@ class DesModel:
     def run(self, n, precision=abi.DES_FP64):
         self._check(self.lib.des_run(self.ctx, n, precision))
 
-    PASS_NAMES = ("pass_a", "pass_b", "pass_c", "pass_d", "pass_e", "facet", "pass_f", "dt")
+    PASS_NAMES = ("pass_a", "pass_b", "pass_c", "pass_d", "pass_e", "facet", "pass_f", "dt", "pass_c_ret")
 
     def run_timed(self, n, precision=abi.DES_FP64):
         """des_run with per-pass HIP-event timing: {pass: (total_ms, launches)}."""
-        ms = (C.c_double * 8)()
-        cnt = (C.c_int * 8)()
+        ms = (C.c_double * 10)()
+        cnt = (C.c_int * 10)()
         self._check(self.lib.des_run_timed(self.ctx, n, precision, ms, cnt))
-        return {self.PASS_NAMES[k]: (ms[k], cnt[k]) for k in range(8) if cnt[k]}
+        return {self.PASS_NAMES[k]: (ms[k], cnt[k]) for k in range(len(self.PASS_NAMES)) if cnt[k]}
+
+    def set_tuning(self, variant):
+        self._check(self.lib.des_set_tuning(self.ctx, variant))
 
     def sync(self):
         self._check(self.lib.des_sync(self.ctx))

@@ -213,8 +213,9 @@ int des_run(des_ctx *ctx, int nsteps, int precision);
 /* des_run with the duration of each fused pass measured by HIP events on the
    context's stream: ms[k] accumulates the milliseconds and counts[k] the
    launches of pass k (0..4 = passes A..E, 5 = facet tractions, 6 = pass F,
-   7 = compute_dt).  Measurement aid with no counterpart in the reference. */
-int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[8], int counts[8]);
+   7 = compute_dt, 8 = pass C's deferred Mohr-Coulomb returns; 9 unused).
+   Measurement aid with no counterpart in the reference. */
+int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[10], int counts[10]);
 
 /* Wait for all work queued on the context's stream; returns the first device
    error the step detected (dt <= 0, missing edge vector). */
@@ -224,6 +225,8 @@ int des_sync(des_ctx *ctx);
    of des_run chose, -1 before a run of 16 or more steps, and the milliseconds
    it measured for each.  No counterpart in the reference. */
 int des_get_tuning(des_ctx *ctx, int *variant, double ms[4]);
+/* Fix the pass-C variant (0..3), or -1 to let the next long run autotune. */
+int des_set_tuning(des_ctx *ctx, int variant);
 
 /* ---- differentiated run (no counterpart in the reference) ------------------
 
