@@ -135,12 +135,16 @@ class Profiler:
     def __init__(self):
         import ctypes
         self.lib = None
+        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
         for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
-            try:
-                self.lib = ctypes.CDLL(name)
+            for path in (name, os.path.join(rocm, "lib", name)):
+                try:
+                    self.lib = ctypes.CDLL(path)
+                    break
+                except OSError:
+                    continue
+            if self.lib is not None:
                 break
-            except OSError:
-                continue
         if self.lib is not None:
             self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
             self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]

@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <initializer_list>
+#include <tuple>
 #include <vector>
 
 #include "des_amd.h"
@@ -376,11 +378,15 @@ template <int ND> __device__ __forceinline__ void force_contrib(const DevState &
     }
 }
 
+// Element-major contributions (96 contiguous bytes per tetrahedron, six
+// 16-byte stores); the node gather reads them through the CSR entry e*NPE+i.
+// (Writing them node-major, to the CSR slots, makes the gather contiguous but
+// the scattered stores cost the element pass more than the gather saves.)
 template <int ND> __device__ __forceinline__ void store_force_contrib(const DevState &s, int e,
                                                                     const double out[Dim<ND>::NPE][ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
-    double *w = s.ework + (long)e * NPE * ND;
+    double *w = s.fwork + (long)e * NPE * ND;
     if constexpr (ND == 3) {
         double2 *w2 = reinterpret_cast<double2 *>(w);
         w2[0] = make_double2(out[0][0], out[0][1]);
@@ -518,14 +524,14 @@ template <int ND> __device__ __forceinline__ void node_force(const DevState &s, 
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) c[j][d] = s.ework[(long)en[j] * ND + d];
+                for (int d = 0; d < ND; ++d) c[j][d] = s.fwork[(long)en[j] * ND + d];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) f[d] -= c[j][d];
         }
         for (; p < p1; ++p) {
-            const double *c = s.ework + (long)s.csr_ent[p] * ND;
+            const double *c = s.fwork + (long)s.csr_ent[p] * ND;
 #pragma unroll
             for (int d = 0; d < ND; ++d) f[d] -= c[d];
         }
@@ -1141,12 +1147,11 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
         load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
         shape_fn<ND>(x, s.volume[e], sh);
     }
-    double w[3] = {0, 0, 0};
+    double w[3];
     const bool rot = s.sc->rot_pending;
-    if (rot) spin<ND>(v, sh, w);
     const double dtp = s.sc->dt_prev;
     double edot[NSTR];
-    strain_rate<ND>(v, sh, edot);
+    strain_rate_spin<ND>(v, sh, edot, w, rot);
     double edv = 0;
 #pragma unroll
     for (int i = 0; i < NPE; ++i) edv += aux[i];
@@ -1820,6 +1825,15 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
     H2D(d_ent, ent.data(), ent.size());
     s.csr_off = d_off;
     s.csr_ent = d_ent;
+    {   // CSR position of every entry, for the node-major force contributions
+        std::vector<int> slot((size_t)E * npe, 0);
+        for (int q = 0; q < off[N]; ++q) slot[ent[q]] = q;
+        int *d_slot;
+        ALLOC(d_slot, slot.size());
+        H2D(d_slot, slot.data(), slot.size());
+        s.slot = d_slot;
+        ALLOC(s.fwork, (size_t)E * npe * nd);
+    }
     // boundary facets; the loaded ones get a per-node list in reference order
     // (boundary by boundary, facets ascending)
     if (m->nbfacet > 0) {

@@ -60,7 +60,9 @@ struct DevState {
     double *strain_rate, *strain, *stress;        // NSTR x E
     double *shp;                                  // ND x NPE x E
     const double *elem_visc, *elem_cohesion;      // E or null
-    double *ework;                                // E x NPE x ND contributions
+    double *ework;                                // E x NPE scalar contributions (temperature)
+    double *fwork;                                // E x NPE x ND force contributions
+    const int *slot;                              // E x NPE: CSR position of entry e*NPE+i
     double *erec;                                 // E x 8 packed records (fused pass A -> B)
     double *escal;                                // 4 x E element scalars
     double *ftrac;                                // nbf x ND facet tractions
@@ -306,6 +308,51 @@ template <int ND> __device__ __forceinline__ void strain_rate(const double v[Dim
         s[2] = 0;
 #pragma unroll
         for (int i = 0; i < NPE; ++i) s[2] += 0.5 * (v[i][0] * sz[i] + v[i][ND - 1] * sx[i]);
+    }
+}
+
+// strain_rate and, when with_spin, the spin of rotate_stress from the same
+// products, each sum in the order of its own function (bitwise equal to
+// calling both)
+template <int ND> __device__ __forceinline__ void strain_rate_spin(const double v[Dim<ND>::NPE][ND],
+                                                                 const double sh[ND][Dim<ND>::NPE],
+                                                                 double s[Dim<ND>::NSTR], double w[3],
+                                                                 bool with_spin)
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    const double *sx = sh[0], *sz = sh[ND - 1];
+    if constexpr (ND == 3) {
+        const double *sy = sh[1];
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, w3 = 0, w4 = 0, w5 = 0;
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) {
+            const double xy = v[i][0] * sy[i], yx = v[i][1] * sx[i];
+            const double xz = v[i][0] * sz[i], zx = v[i][2] * sx[i];
+            const double yz = v[i][1] * sz[i], zy = v[i][2] * sy[i];
+            s0 += v[i][0] * sx[i];
+            s1 += v[i][1] * sy[i];
+            s2 += v[i][2] * sz[i];
+            s3 += 0.5 * (xy + yx);
+            s4 += 0.5 * (xz + zx);
+            s5 += 0.5 * (yz + zy);
+            w3 += 0.5 * (xy - yx);
+            w4 += 0.5 * (xz - zx);
+            w5 += 0.5 * (yz - zy);
+        }
+        s[0] = s0; s[1] = s1; s[2] = s2; s[3] = s3; s[4] = s4; s[5] = s5;
+        w[0] = with_spin ? w3 : 0; w[1] = with_spin ? w4 : 0; w[2] = with_spin ? w5 : 0;
+    } else {
+        double s0 = 0, s1 = 0, s2 = 0, w2 = 0;
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) {
+            const double xz = v[i][0] * sz[i], zx = v[i][1] * sx[i];
+            s0 += v[i][0] * sx[i];
+            s1 += v[i][1] * sz[i];
+            s2 += 0.5 * (xz + zx);
+            w2 += 0.5 * (zx - xz);
+        }
+        s[0] = s0; s[1] = s1; s[2] = s2;
+        w[0] = with_spin ? w2 : 0; w[1] = 0; w[2] = 0;
     }
 }
 
