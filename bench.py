@@ -37,9 +37,16 @@ PASS_BYTES_3D = {
 # mixed precision: FP32 deviator/strain/plastic strain in passes C and E
 PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
 
+# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for
+# a plain 1 GiB copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved);
+# measured on this workload, the command in profiles/.
+TRAFFIC = {}
+
 WORKLOADS = {
     "3d-fp64": ("3d-fp64 extension box 100x100x10 km, EVP Mohr-Coulomb", 1_000_000),
     "3d-mixed": ("3d-mixed box 200x200x20 km, per-element viscosity and cohesion, EVP", 8_000_000),
+    "adjoint": ("adjoint box 50x50x10 km, maxwell, per-element log10 viscosity U[22,24]", None),
     "ensemble": ("ensemble of 256 box models (128 2D with 1k-50k triangles, 128 3D with 10k-200k "
                  "tetrahedra, log-uniform sizes, per-model extension rate, friction angle and "
                  "weak-seed position, seed 6), mixed precision", None),
@@ -127,37 +134,6 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-class Profiler:
-    """roctxProfilerPause / Resume from ROCm's ROCTx library: a profiler that
-    honours them (rocprofv3) records only the primary measurement; without a
-    profiler the calls do nothing."""
-
-    def __init__(self):
-        import ctypes
-        self.lib = None
-        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
-            for path in (name, os.path.join(rocm, "lib", name)):
-                try:
-                    self.lib = ctypes.CDLL(path)
-                    break
-                except OSError:
-                    continue
-            if self.lib is not None:
-                break
-        if self.lib is not None:
-            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-
-
 def adjoint_timing(nsteps, window, nelem):
     """The differentiated run of the visco-elastic adjoint configuration:
     plain forward, checkpointed forward and reverse sweep, ms per step."""
@@ -187,6 +163,20 @@ def adjoint_timing(nsteps, window, nelem):
             "gradient_over_forward": (t_fwd + t_bwd) / t_plain}
 
 
+def bench_adjoint(args):
+    """The differentiated run (visco-elastic adjoint configuration, 200 k tets,
+    2,000 steps, checkpoint every 100): gradient cost over forward cost."""
+    adj = adjoint_timing(args.adjoint_steps, 100, args.adjoint_nelem)
+    out = {
+        "metric": METRIC, "value": adj["gradient_over_forward"], "unit": "x forward time",
+        "n_gpus": 1, "steps": args.adjoint_steps, "warmup": 0,
+        "ms_per_step": adj["checkpointed_forward_ms_per_step"] + adj["backward_ms_per_step"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": {"workload": adj["workload"], "window": 100}, "adjoint": adj,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,8 +187,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-nelem", type=int, default=200_000)
     ap.add_argument("--ensemble-scale", type=float, default=1.0)
     ap.add_argument("--seq-steps", type=int, default=100)
     args = ap.parse_args()
@@ -208,40 +198,44 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     from dynearthsol_amd import abi, model as M
     from dynearthsol_amd.lib import DesModel
 
-    prof = Profiler()
-    prof.pause()
     desc, ne_default = WORKLOADS[args.workload]
     nelem = args.nelem or ne_default
     primary = abi.DES_MIXED if args.workload == "3d-mixed" else abi.DES_FP64
     m = M.config_benchmark(args.workload, nelem_target=nelem)
     E, N = m.nelem, m.nnode
-    # two identical models: g runs the timed steps, its twin the same steps
-    # under per-pass HIP events (the step is deterministic, so both see the
-    # same states); the profiler records exactly these two runs
+    # Two identical models.  g's warm-up and the twin's warm-up and timed
+    # steps run under per-pass HIP events; g's timed steps run plain (the
+    # timing) and see the same states as the twin's (the step is
+    # deterministic).  So every launch of the dominant kernel in this process
+    # is either timed by events or mirrored by an event-timed twin launch: the
+    # average below is the one a kernel trace of this command reports.
     g, twin = DesModel(m), DesModel(m)
     print(f"# {args.workload}: {E} elements, {N} nodes; warmup {args.warmup}", file=sys.stderr, flush=True)
-    g.run(args.warmup, primary)
+    warm_g = g.run_timed(args.warmup, primary) if args.warmup else {}
     twin.set_tuning(g.tuning()[0])
-    twin.run(args.warmup, primary)
+    warm_t = twin.run_timed(args.warmup, primary) if args.warmup else {}
     g.sync()
     twin.sync()
-    prof.resume()
     secs = timed(g, args.steps, primary)
     passes = twin.run_timed(args.steps, primary)
     twin.sync()
-    prof.pause()
     ms_per_step = 1e3 * secs / args.steps
     value = E * args.steps / secs
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
     avg = {k: ms / max(c, 1) for k, (ms, c) in passes.items()}
     table = PASS_BYTES_3D_MIXED if primary == abi.DES_MIXED else PASS_BYTES_3D
     dom = max((k for k in avg if k in table), key=lambda k: avg[k])
+    tot_ms = sum(r.get(dom, (0, 0))[0] for r in (warm_g, warm_t)) + 2 * passes[dom][0]
+    tot_n = sum(r.get(dom, (0, 0))[1] for r in (warm_g, warm_t)) + 2 * passes[dom][1]
+    proc_avg = tot_ms / tot_n
     be, bn = table[dom]
-    achieved = (be * E + bn * N) / (avg[dom] * 1e-3) / 1e9
+    achieved = (be * E + bn * N) / (proc_avg * 1e-3) / 1e9
     step_bytes = sum(be_ * E + bn_ * N for be_, bn_ in table.values())
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
     kernel = {"pass_c": "k_pass_c" if primary == abi.DES_FP64 else "k_pass_c_mixed"}.get(dom, "k_" + dom)
@@ -266,10 +260,6 @@ def main():
     g.close()
     twin.close()
 
-    adj = None
-    if not args.no_adjoint:
-        adj = adjoint_timing(args.adjoint_steps, 100, 200_000)
-
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(M.config_benchmark(args.workload, nelem_target=nelem), args.cpu_steps)
@@ -289,14 +279,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
                    "precision": "fp64" if primary == abi.DES_FP64 else "mixed"},
-        "roofline": {"bound": "hbm", "kernel": kernel, "pass": dom, "avg_ms": avg[dom],
+        "roofline": {"bound": "hbm", "kernel": kernel, "pass": dom, "avg_ms": proc_avg,
+                     "launches": tot_n, "timed_window_avg_ms": avg[dom],
+                     "bytes_per_launch": be * E + bn * N,
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(args.workload)},
         "step_hbm": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},
         "fp64_vs_mixed": compare,
-        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -54,15 +54,16 @@ def test_mixed_close_to_fp64(name):
 
 def test_mixed_chunks_and_state_roundtrip():
     """Mixed chunks leave an FP64 state that continues seamlessly: the
-    conversion at a chunk boundary changes the state by FP64 rounding only
-    (a smooth model, where that rounding is not amplified)."""
+    conversion at a chunk boundary re-splits the FP64 stress into pressure
+    and FP32 deviator, which moves the deviator by at most one FP32 ulp (a
+    smooth model, where that is not amplified; measured 3e-8)."""
     m = SC.build("maxwell3")
     g1, g2 = _pair(m)
     g1.run(40, abi.DES_MIXED)
     g2.run(15, abi.DES_MIXED)
     g2.run(25, abi.DES_MIXED)
     e = _errors(g1.download(), g2.download())
-    assert e["vel"] < 1e-8 and e["stress"] < 1e-8, e
+    assert e["vel"] < 1e-6 and e["stress"] < 1e-6, e
     dt, t, st = g2.scalars()
     assert st == 40
 
