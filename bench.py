@@ -53,6 +53,14 @@ WORKLOADS = {
 }
 
 
+def kernel_name(precision, p):
+    """The kernel that runs fused pass p at this precision."""
+    from dynearthsol_amd import abi
+    if precision == abi.DES_MIXED and p in ("pass_c", "pass_e"):
+        return f"k_{p}_mixed"
+    return "k_dt_partial+k_dt_final" if p == "dt" else "k_" + p
+
+
 def cpu_baseline(model, nsteps):
     """The reference's own step functions (oracle/_ref, OpenMP build) timed on
     this host over a bounded sample of the same workload."""
@@ -209,44 +217,58 @@ def main():
     primary = abi.DES_MIXED if args.workload == "3d-mixed" else abi.DES_FP64
     m = M.config_benchmark(args.workload, nelem_target=nelem)
     E, N = m.nelem, m.nnode
-    # Two identical models.  g's warm-up and the twin's warm-up and timed
-    # steps run under per-pass HIP events; g's timed steps run plain (the
-    # timing) and see the same states as the twin's (the step is
-    # deterministic).  So every launch of the dominant kernel in this process
-    # is either timed by events or mirrored by an event-timed twin launch: the
-    # average below is the one a kernel trace of this command reports.
-    g, twin = DesModel(m), DesModel(m)
+    # Two identical models per precision.  g's warm-up and the twin's warm-up
+    # and timed steps run under per-pass events that the dispatches timestamp
+    # (kernel execution time); g's timed steps run plain (the timing) and see
+    # the same states as the twin's (the step is deterministic).  So every
+    # launch of a kernel in this process is either event-timed or mirrored by
+    # an event-timed twin launch, and the per-kernel averages below are the
+    # ones a kernel trace of this command reports.
+    kernel_totals = {}
+
+    def book(precision, passes, times=1):
+        for p, (ms, c) in passes.items():
+            k = kernel_name(precision, p)
+            t = kernel_totals.setdefault(k, [0.0, 0])
+            t[0] += times * ms
+            t[1] += times * c
+
+    def timed_pair(precision, tuning=None):
+        a, b = DesModel(m), DesModel(m)
+        if tuning is not None:
+            a.set_tuning(tuning)
+        wa = a.run_timed(args.warmup, precision) if args.warmup else {}
+        b.set_tuning(a.tuning()[0])
+        wb = b.run_timed(args.warmup, precision) if args.warmup else {}
+        a.sync()
+        b.sync()
+        secs = timed(a, args.steps, precision)
+        passes = b.run_timed(args.steps, precision)
+        b.sync()
+        book(precision, wa)
+        book(precision, wb)
+        book(precision, passes, times=2)
+        b.close()
+        return a, secs, passes
+
     print(f"# {args.workload}: {E} elements, {N} nodes; warmup {args.warmup}", file=sys.stderr, flush=True)
-    warm_g = g.run_timed(args.warmup, primary) if args.warmup else {}
-    twin.set_tuning(g.tuning()[0])
-    warm_t = twin.run_timed(args.warmup, primary) if args.warmup else {}
-    g.sync()
-    twin.sync()
-    secs = timed(g, args.steps, primary)
-    passes = twin.run_timed(args.steps, primary)
-    twin.sync()
+    g, secs, passes = timed_pair(primary)
     ms_per_step = 1e3 * secs / args.steps
     value = E * args.steps / secs
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
     avg = {k: ms / max(c, 1) for k, (ms, c) in passes.items()}
     table = PASS_BYTES_3D_MIXED if primary == abi.DES_MIXED else PASS_BYTES_3D
     dom = max((k for k in avg if k in table), key=lambda k: avg[k])
-    tot_ms = sum(r.get(dom, (0, 0))[0] for r in (warm_g, warm_t)) + 2 * passes[dom][0]
-    tot_n = sum(r.get(dom, (0, 0))[1] for r in (warm_g, warm_t)) + 2 * passes[dom][1]
-    proc_avg = tot_ms / tot_n
     be, bn = table[dom]
-    achieved = (be * E + bn * N) / (proc_avg * 1e-3) / 1e9
     step_bytes = sum(be_ * E + bn_ * N for be_, bn_ in table.values())
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
-    kernel = {"pass_c": "k_pass_c" if primary == abi.DES_FP64 else "k_pass_c_mixed"}.get(dom, "k_" + dom)
+    kernel = kernel_name(primary, dom)
 
     # the other precision from the same initial state, over the same steps
     compare = None
     if not args.no_mixed:
         other = abi.DES_FP64 if primary == abi.DES_MIXED else abi.DES_MIXED
-        g2 = DesModel(m)
-        g2.run(args.warmup, other)
-        t2 = timed(g2, args.steps, other)
+        g2, t2, _ = timed_pair(other)
         names = ["vel", "stress", "coord", "plstrain"]
         a, b = g2.download(names), g.download(names)
         t64, t32 = (secs, t2) if primary == abi.DES_FP64 else (t2, secs)
@@ -258,7 +280,9 @@ def main():
                    "mixed_rel_err_vs_fp64": {k: rel(a[k], b[k]) for k in names}}
         g2.close()
     g.close()
-    twin.close()
+    tot_ms, tot_n = kernel_totals[kernel]
+    proc_avg = tot_ms / tot_n
+    achieved = (be * E + bn * N) / (proc_avg * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:

@@ -13,6 +13,7 @@
 // element order -- the reference's serial summation order -- so no atomics and
 // bitwise run-to-run determinism.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -378,28 +379,20 @@ template <int ND> __device__ __forceinline__ void force_contrib(const DevState &
     }
 }
 
-// Element-major contributions (96 contiguous bytes per tetrahedron, six
-// 16-byte stores); the node gather reads them through the CSR entry e*NPE+i.
-// (Writing them node-major, to the CSR slots, makes the gather contiguous but
-// the scattered stores cost the element pass more than the gather saves.)
+// Node-major contributions: element e's contribution to its i-th node goes to
+// that node's CSR slot slot[e*NPE+i], so the node gather reads one contiguous
+// run per node.  With the mesh in Morton order an element's slots are close,
+// and the scattered stores cost the element pass less than the contiguous
+// gather saves (measured against 96-byte element-major records).
 template <int ND> __device__ __forceinline__ void store_force_contrib(const DevState &s, int e,
                                                                     const double out[Dim<ND>::NPE][ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
-    double *w = s.fwork + (long)e * NPE * ND;
-    if constexpr (ND == 3) {
-        double2 *w2 = reinterpret_cast<double2 *>(w);
-        w2[0] = make_double2(out[0][0], out[0][1]);
-        w2[1] = make_double2(out[0][2], out[1][0]);
-        w2[2] = make_double2(out[1][1], out[1][2]);
-        w2[3] = make_double2(out[2][0], out[2][1]);
-        w2[4] = make_double2(out[2][2], out[3][0]);
-        w2[5] = make_double2(out[3][1], out[3][2]);
-    } else {
 #pragma unroll
-        for (int i = 0; i < NPE; ++i)
+    for (int i = 0; i < NPE; ++i) {
+        double *w = s.fwork + (long)s.slot[e * NPE + i] * ND;
 #pragma unroll
-            for (int d = 0; d < ND; ++d) w[i * ND + d] = out[i][d];
+        for (int d = 0; d < ND; ++d) w[d] = out[i][d];
     }
 }
 
@@ -517,21 +510,18 @@ template <int ND> __device__ __forceinline__ void node_force(const DevState &s, 
         const int p1 = s.csr_off[n + 1];
         int p = s.csr_off[n];
         for (; p + 4 <= p1; p += 4) {   // four entries in flight, summed in order
-            int en[4];
             double c[4][ND];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) en[j] = s.csr_ent[p + j];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int d = 0; d < ND; ++d) c[j][d] = s.fwork[(long)en[j] * ND + d];
+                for (int d = 0; d < ND; ++d) c[j][d] = s.fwork[(long)(p + j) * ND + d];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) f[d] -= c[j][d];
         }
         for (; p < p1; ++p) {
-            const double *c = s.fwork + (long)s.csr_ent[p] * ND;
+            const double *c = s.fwork + (long)p * ND;
 #pragma unroll
             for (int d = 0; d < ND; ++d) f[d] -= c[d];
         }
@@ -592,18 +582,13 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_update_velocity(Dev
     for (int d = 0; d < ND; ++d) s.vel[d * s.N + n] += dt * s.force[d * s.N + n] / m;
 }
 
-struct Vec3 {
-    double v[3];
-    int ok;
-};
-
-// the slant boundaries n0..n3 of apply_vbcs (bc.cxx:350-436); out of line and
-// by value because no box model has them
-template <int ND> __device__ __noinline__ Vec3 vbc_slant(const DevState &s, unsigned flag, Vec3 in)
+// the slant boundaries n0..n3 of apply_vbcs (bc.cxx:350-436); every index
+// into v is static, so the node's velocity stays in registers
+template <int ND> __device__ __forceinline__ bool vbc_slant(const DevState &s, unsigned flag, double v[ND])
 {
     const int *T = s.P->vbc_types;
     const double *V = s.P->vbc_values;
-    double *v = in.v;
+    bool ok = true;
     for (int ib = IBOUNDN0; ib <= IBOUNDN3; ib++) {
         if (!(flag & (1u << ib))) continue;
         const double *nv = s.bnormals + ib * ND;
@@ -616,20 +601,30 @@ template <int ND> __device__ __noinline__ Vec3 vbc_slant(const DevState &s, unsi
             const double target = (T[ib] == 1) ? V[ib] : V[ib] * fac;
             if (flag == (1u << ib)) {
                 double vn = 0;
-                for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
-                for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
+#pragma unroll
+                for (int d = 0; d < ND; d++)
+                    if (d < nh) vn += v[d] * nv[d];
+#pragma unroll
+                for (int d = 0; d < ND; d++)
+                    if (d < nh) v[d] += (target - vn) * nv[d];
             } else {
                 for (int ic = 0; ic < ib; ic++) {
                     if (!(flag & (1u << ic))) continue;
                     if (T[ic] == 0) {
                         double vn = 0;
-                        for (int d = 0; d < nh; d++) vn += v[d] * nv[d];
-                        for (int d = 0; d < nh; d++) v[d] += (target - vn) * nv[d];
+#pragma unroll
+                        for (int d = 0; d < ND; d++)
+                            if (d < nh) vn += v[d] * nv[d];
+#pragma unroll
+                        for (int d = 0; d < ND; d++)
+                            if (d < nh) v[d] += (target - vn) * nv[d];
                     } else if (T[ic] == 1) {
-                        if (!s.edge_valid[ic * DES_NBDRY + ib]) { in.ok = 0; continue; }
+                        if (!s.edge_valid[ic * DES_NBDRY + ib]) { ok = false; continue; }
                         const double *edge = s.edge_vectors + (ic * DES_NBDRY + ib) * ND;
                         double ve = 0;
+#pragma unroll
                         for (int d = 0; d < ND; d++) ve += v[d] * edge[d];
+#pragma unroll
                         for (int d = 0; d < ND; d++) v[d] = ve * edge[d];
                     }
                 }
@@ -637,16 +632,18 @@ template <int ND> __device__ __noinline__ Vec3 vbc_slant(const DevState &s, unsi
             break;
         }
         case 3:
+#pragma unroll
             for (int d = 0; d < ND; d++) v[d] = V[ib] * nv[d];
             break;
         case 13:
             fac = 1 / sqrt(1 - nv[ND - 1] * nv[ND - 1]);
+#pragma unroll
             for (int d = 0; d < ND - 1; d++) v[d] = V[ib] * fac * nv[d];
             v[ND - 1] = 0;
             break;
         }
     }
-    return in;
+    return ok;
 }
 
 // apply_vbcs for one node (bc.cxx:179-490); returns false if an edge vector
@@ -699,16 +696,8 @@ template <int ND> __device__ __forceinline__ bool vbc_node(const DevState &s, un
             }
         }
     }
-    if (flag & 0x3C0u) {     // BOUNDN0..BOUNDN3
-        Vec3 in;
-        in.ok = 1;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) in.v[d] = d < ND ? v[d] : 0.0;
-        Vec3 out = vbc_slant<ND>(s, flag, in);
-#pragma unroll
-        for (int d = 0; d < ND; ++d) v[d] = out.v[d];
-        if (!out.ok) ok = false;
-    }
+    if (flag & 0x3C0u)       // BOUNDN0..BOUNDN3
+        if (!vbc_slant<ND>(s, flag, v)) ok = false;
     if (T[4] == 0 && T[5] == 0) return ok;
     if (flag & BOUNDZ0) {
         switch (T[4]) {
@@ -910,7 +899,7 @@ template <int ND> __device__ __forceinline__ double elem_quality(const DevState 
     }
 }
 
-// (quality, element) minimum over the block, smaller index on ties; slot b
+// (quality, host element index) minimum over the block, smaller index on ties; slot b
 __device__ __forceinline__ void block_argmin_store(DevState &s, int slot, double q, int qi)
 {
     __shared__ double sq[BLOCK];
@@ -941,7 +930,8 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial(Dev
     int qi = 0x7fffffff;
     for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK) {
         const double quality = elem_quality<ND>(s, e);
-        if (quality < q) { q = quality; qi = e; }
+        const int h = s.ehost[e];
+        if (quality < q || (quality == q && h < qi)) { q = quality; qi = h; }
     }
     block_argmin_store(s, blockIdx.x, q, qi);
 }
@@ -955,7 +945,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_quality_partial_b(D
     int qi = 0x7fffffff;
     if (e < s.eblk_lim[blk]) {
         q = elem_quality<ND>(s, e);
-        qi = e;
+        qi = s.ehost[e];
     }
     block_argmin_store(s, blk, q, qi);
 }
@@ -979,23 +969,18 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_elem_diagnostics(De
 
 // ---- bad_mesh_quality's first two checks (remeshing.cxx:1893-1918): the
 // first element smaller than smallest_vol and the first bottom node farther
-// than dist from the initial bottom depth, as block minima of their indices
+// than dist from the initial bottom depth, as block minima of their host indices
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_badmesh_partial(DevState s, double smallest_vol,
                                                                            int check_bottom, double bottom,
                                                                            double dist)
 {
     int first_e = 0x7fffffff, first_n = 0x7fffffff;
     for (int e = blockIdx.x * BLOCK + threadIdx.x; e < s.E; e += gridDim.x * BLOCK)
-        if (s.volume[e] < smallest_vol) {
-            first_e = min(first_e, e);
-            break;
-        }
+        if (s.volume[e] < smallest_vol) first_e = min(first_e, s.ehost[e]);
     if (check_bottom) {
         for (int n = blockIdx.x * BLOCK + threadIdx.x; n < s.N; n += gridDim.x * BLOCK)
-            if ((s.bcflag[n] & BOUNDZ0) && fabs(s.coord[(ND - 1) * s.N + n] - bottom) > dist) {
-                first_n = min(first_n, n);
-                break;
-            }
+            if ((s.bcflag[n] & BOUNDZ0) && fabs(s.coord[(ND - 1) * s.N + n] - bottom) > dist)
+                first_n = min(first_n, s.nhost[n]);
     }
     __shared__ int se[BLOCK], sn[BLOCK];
     se[threadIdx.x] = first_e;
@@ -1027,9 +1012,20 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_badmesh_partial(Dev
 //   coordinates.
 
 struct FusedFlags {
-    int use_stored_mass;   // first step after a sync: mass/tmass/volume_n from compute_mass
-    int store_aux;         // last step before a sync: keep strain_rate, edvoldt, force
+    int use_stored_mass = 0;   // first step after a sync: mass/tmass/volume_n from compute_mass
+    int store_aux = 0;         // last step before a sync: keep strain_rate, edvoldt, force
+    int step_end = 0;          // pass F also ends the step: time, steps, dt_prev, rotation flag
 };
+
+// the scalar bookkeeping at the end of a step (var.time, var.steps), and the
+// Jaumann rotation owed by the next pass C when the rheology is elastic
+__device__ __forceinline__ void step_end_scalars(DevScalars *sc, int rheol_type)
+{
+    sc->time += sc->dt;
+    sc->steps += 1;
+    sc->dt_prev = sc->dt;
+    sc->rot_pending = (rheol_type & DES_RH_ELASTIC) ? 1 : 0;
+}
 
 // Packed per-element record written by pass A and gathered by pass B: the
 // thermal contributions to the element's nodes, tr(edot) V, the element's
@@ -1317,6 +1313,10 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
 {
     BIND_NODE(s, n);
     const double dt = s.sc->dt;
+    // one lane per model ends the step; nothing in this pass reads what it updates
+    if (fl.step_end && threadIdx.x == 0 &&
+        (s.nblk_model ? (blk_ == 0 || s.nblk_model[blk_ - 1] != s.nblk_model[blk_]) : blk_ == 0))
+        step_end_scalars(s.sc, s.P->rheol_type);
     double f[ND], v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = s.vel[d * s.N + n];
@@ -1492,27 +1492,6 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pack_nrec(DevState 
 }
 
 // step bookkeeping after pass F: var.time, var.steps, the rotation to defer
-__global__ void k_step_end(DevState s, int rotate)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    DevScalars *sc = s.sc;
-    sc->time += sc->dt;
-    sc->steps += 1;
-    sc->dt_prev = sc->dt;
-    sc->rot_pending = rotate;
-}
-
-// the same for every batched model, each with its own rheology
-__global__ void k_step_end_b(DevState s)
-{
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= s.nmodels) return;
-    DevScalars *sc = s.scb + m;
-    sc->time += sc->dt;
-    sc->steps += 1;
-    sc->dt_prev = sc->dt;
-    sc->rot_pending = (s.Pb[m].rheol_type & DES_RH_ELASTIC) ? 1 : 0;
-}
 
 __global__ void k_clear_rotation_b(DevState s)
 {
@@ -1549,6 +1528,9 @@ struct des_ctx {
     des_params P;
     int nd, npe, nstr, N, E, nbf;
     hipStream_t stream = nullptr;
+    // start/stop events handed to the next launches (timed runs only): the
+    // dispatch itself timestamps them, so they time the kernel's execution
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
     DevState s{};
     std::vector<void *> allocs;
     std::string err;
@@ -1559,6 +1541,7 @@ struct des_ctx {
     double *elem0 = nullptr;       // bulkm(0), rho(0)
     int steps = 0;                 // host mirror of var.steps
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
+    std::vector<int> eh, nh;       // device element / node -> host index (model-local in a group)
     std::vector<int> h_conn;       // host copies for output frames
     std::vector<unsigned> h_bcflag;
     int passc_variant = -1;        // waves-per-EU variant of pass C chosen by the autotuner
@@ -1623,8 +1606,10 @@ int check_launch(des_ctx *ctx)
 
 #define LAUNCH_ND(kern, grid, ...)                                                        \
     do {                                                                                  \
-        if (ctx->nd == 3) hipLaunchKernelGGL(kern<3>, dim3(grid), dim3(BLOCK), 0, ctx->stream, __VA_ARGS__); \
-        else hipLaunchKernelGGL(kern<2>, dim3(grid), dim3(BLOCK), 0, ctx->stream, __VA_ARGS__); \
+        if (ctx->nd == 3)                                                                 \
+            hipExtLaunchKernelGGL(kern<3>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, __VA_ARGS__); \
+        else                                                                              \
+            hipExtLaunchKernelGGL(kern<2>, dim3(grid), dim3(BLOCK), 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, __VA_ARGS__); \
         int rc_ = check_launch(ctx);                                                      \
         if (rc_) return rc_;                                                              \
     } while (0)
@@ -1639,15 +1624,18 @@ bool facet_loaded(const des_params &P, int i)
     return true;
 }
 
-// host AoS [n][k] <-> device SoA [k][n]
-// host AoS [n][k] <-> device SoA [k][stride], rows off .. off+n-1
-int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0)
+// host AoS [n][k] <-> device SoA [k][stride], rows off .. off+n-1; device
+// row off + i holds host row perm[i] (i without perm)
+int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0,
+               const int *perm = nullptr)
 {
     if (!src) return DES_OK;
     if (stride < 0) stride = n;
     std::vector<double> tmp((size_t)n * k);
-    for (long i = 0; i < n; ++i)
-        for (int j = 0; j < k; ++j) tmp[(size_t)j * n + i] = src[(size_t)i * k + j];
+    for (long i = 0; i < n; ++i) {
+        const size_t h = perm ? (size_t)perm[i] : (size_t)i;
+        for (int j = 0; j < k; ++j) tmp[(size_t)j * n + i] = src[h * k + j];
+    }
     for (int j = 0; j < k; ++j)
         HIPCHK(hipMemcpyAsync(dst + j * stride + off, tmp.data() + (size_t)j * n, n * sizeof(double),
                               hipMemcpyHostToDevice, ctx->stream));
@@ -1655,7 +1643,8 @@ int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long
     return DES_OK;
 }
 
-int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0)
+int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0,
+                 const int *perm = nullptr)
 {
     if (!dst) return DES_OK;
     if (stride < 0) stride = n;
@@ -1664,8 +1653,10 @@ int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, lo
         HIPCHK(hipMemcpyAsync(tmp.data() + (size_t)j * n, src + j * stride + off, n * sizeof(double),
                               hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (long i = 0; i < n; ++i)
-        for (int j = 0; j < k; ++j) dst[(size_t)i * k + j] = tmp[(size_t)j * n + i];
+    for (long i = 0; i < n; ++i) {
+        const size_t h = perm ? (size_t)perm[i] : (size_t)i;
+        for (int j = 0; j < k; ++j) dst[h * k + j] = tmp[(size_t)j * n + i];
+    }
     return DES_OK;
 }
 
@@ -1758,9 +1749,115 @@ int flush_deferred(des_ctx *ctx)
 
 namespace {
 
+// ---- device numbering of a mesh
+//
+// Elements and nodes live on the device in Morton (Z-curve) order of their
+// centroids and positions: the 64 elements of a wavefront share nodes and the
+// 64 nodes of a wavefront share elements, so the gathers of every pass touch
+// few cache lines, and the contiguous range each XCD runs is a compact region
+// whose data its L2 holds.  The host numbering stays the interface: every
+// transfer permutes rows, every index handed back is a host index, and each
+// node's CSR lists its elements in ascending host order, so every sum runs in
+// the reference's order and the results are bitwise those of the host
+// numbering.
+int g_mesh_order = 1;    // 1: Morton order, 0: host numbering (des_set_mesh_order)
+
+// the bits of v (below 2^21) spread to every nd-th bit
+uint64_t spread_bits(uint64_t v, int nd)
+{
+    uint64_t r = 0;
+    for (int b = 0; b < (nd == 3 ? 21 : 31); ++b) r |= ((v >> b) & 1ull) << (b * nd);
+    return r;
+}
+
+// perm[device index] = host index, by Morton key of the n points x[n][nd]
+// (one scale for every axis, so the curve's cells are cubes)
+void morton_order(const double *x, long n, int nd, std::vector<int> &perm)
+{
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (long i = 0; i < n; ++i)
+        for (int d = 0; d < nd; ++d) {
+            lo[d] = std::min(lo[d], x[i * nd + d]);
+            hi[d] = std::max(hi[d], x[i * nd + d]);
+        }
+    double ext = 0;
+    for (int d = 0; d < nd; ++d) ext = std::max(ext, hi[d] - lo[d]);
+    const double cells = (double)((1u << (nd == 3 ? 21 : 31)) - 1);
+    const double scale = ext > 0 ? cells / ext : 0;
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (long i = 0; i < n; ++i) {
+        uint64_t k = 0;
+        for (int d = 0; d < nd; ++d) {
+            double q = std::floor((x[i * nd + d] - lo[d]) * scale);
+            q = std::min(std::max(q, 0.0), cells);
+            k |= spread_bits((uint64_t)q, nd) << d;
+        }
+        key[i] = {k, (int)i};
+    }
+    std::sort(key.begin(), key.end());
+    perm.resize(n);
+    for (long i = 0; i < n; ++i) perm[i] = key[i].second;
+}
+
+// A mesh renumbered into device order, with the maps back to host indices.
+struct OrderedMesh {
+    std::vector<int> eh, nh;            // device element / node -> host index
+    std::vector<int> conn, felem, markers;
+    std::vector<double> coord;
+    std::vector<unsigned> bcflag;
+    des_mesh m;                         // in device order (other arrays: the input's)
+};
+
+void order_mesh(const des_mesh &in, int nd, int nmat, int mode, OrderedMesh &o)
+{
+    const long E = in.nelem, N = in.nnode;
+    const int npe = nd + 1;
+    o.nh.resize(N);
+    o.eh.resize(E);
+    if (mode == 0) {
+        for (long i = 0; i < N; ++i) o.nh[i] = (int)i;
+        for (long i = 0; i < E; ++i) o.eh[i] = (int)i;
+    } else {
+        morton_order(in.coord, N, nd, o.nh);
+        std::vector<double> cen((size_t)E * nd, 0.0);
+        for (long e = 0; e < E; ++e)
+            for (int i = 0; i < npe; ++i)
+                for (int d = 0; d < nd; ++d) cen[e * nd + d] += in.coord[(size_t)in.connectivity[e * npe + i] * nd + d];
+        morton_order(cen.data(), E, nd, o.eh);
+    }
+    std::vector<int> ninv(N), einv(E);
+    for (long i = 0; i < N; ++i) ninv[o.nh[i]] = (int)i;
+    for (long i = 0; i < E; ++i) einv[o.eh[i]] = (int)i;
+    o.conn.resize((size_t)E * npe);
+    for (long e = 0; e < E; ++e)
+        for (int i = 0; i < npe; ++i) o.conn[e * npe + i] = ninv[in.connectivity[(size_t)o.eh[e] * npe + i]];
+    o.coord.resize((size_t)N * nd);
+    o.bcflag.resize(N);
+    for (long n = 0; n < N; ++n) {
+        for (int d = 0; d < nd; ++d) o.coord[n * nd + d] = in.coord[(size_t)o.nh[n] * nd + d];
+        o.bcflag[n] = in.bcflag[o.nh[n]];
+    }
+    o.felem.resize(std::max(in.nbfacet, 0));
+    for (int k = 0; k < in.nbfacet; ++k) o.felem[k] = einv[in.bfacet_elem[k]];
+    if (in.elemmarkers) {
+        o.markers.resize((size_t)E * nmat);
+        for (long e = 0; e < E; ++e)
+            for (int j = 0; j < nmat; ++j) o.markers[e * nmat + j] = in.elemmarkers[(size_t)o.eh[e] * nmat + j];
+    }
+    o.m = in;
+    o.m.connectivity = o.conn.data();
+    o.m.coord = o.coord.data();
+    o.m.bcflag = o.bcflag.data();
+    o.m.bfacet_elem = in.nbfacet > 0 ? o.felem.data() : in.bfacet_elem;
+    o.m.elemmarkers = in.elemmarkers ? o.markers.data() : nullptr;
+}
+
 // One context: a single model (g == nullptr) or a batched group whose models
-// are concatenated into the mesh m, laid out as g describes.
-int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const GroupLayout *g)
+// are concatenated into the mesh m, laid out as g describes.  m is in device
+// order; eh / nh map its elements / nodes to (model-local) host indices, -1
+// on a group's padding.
+int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const GroupLayout *g,
+                const std::vector<int> &eh, const std::vector<int> &nh)
 {
     ctx->P = *p;
     const int nd = ctx->nd = p->ndims;
@@ -1817,6 +1914,21 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
         for (int e = 0; e < E; ++e)
             if (live[e])
                 for (int i = 0; i < npe; ++i) ent[fill[m->connectivity[(size_t)e * npe + i]]++] = e * npe + i;
+        // a node's elements in ascending host order: the reference's summation order
+        for (int n = 0; n < N; ++n)
+            std::sort(ent.begin() + off[n], ent.begin() + off[n + 1],
+                      [&](int a, int b) { return eh[a / npe] < eh[b / npe]; });
+    }
+    ctx->eh = eh;
+    ctx->nh = nh;
+    {
+        int *d_eh, *d_nh;
+        ALLOC(d_eh, E);
+        ALLOC(d_nh, N);
+        H2D(d_eh, eh.data(), E);
+        H2D(d_nh, nh.data(), N);
+        s.ehost = d_eh;
+        s.nhost = d_nh;
     }
     int *d_off, *d_ent;
     ALLOC(d_off, N + 1);
@@ -1832,7 +1944,7 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
         ALLOC(d_slot, slot.size());
         H2D(d_slot, slot.data(), slot.size());
         s.slot = d_slot;
-        ALLOC(s.fwork, (size_t)E * npe * nd);
+        ALLOC(s.fwork, (size_t)std::max(off[N], 1) * nd);
     }
     // boundary facets; the loaded ones get a per-node list in reference order
     // (boundary by boundary, facets ascending)
@@ -1975,9 +2087,6 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
         ctx->nbf_cnt = g->nbf_cnt;
         ctx->has_visc.assign(nmodels, 0);
         ctx->has_coh.assign(nmodels, 0);
-    } else {
-        ctx->h_conn.assign(m->connectivity, m->connectivity + (size_t)E * npe);
-        ctx->h_bcflag.assign(m->bcflag, m->bcflag + N);
     }
 #undef ALLOC
 #undef H2D
@@ -2020,8 +2129,15 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
     if (rc) return rc;
     des_ctx *ctx = new des_ctx();
     *out = ctx;
-    return create_impl(ctx, p, m, nullptr);
+    OrderedMesh om;
+    order_mesh(*m, p->ndims, p->nmat, g_mesh_order, om);
+    if ((rc = create_impl(ctx, p, &om.m, nullptr, om.eh, om.nh))) return rc;
+    ctx->h_conn.assign(m->connectivity, m->connectivity + (size_t)m->nelem * (p->ndims + 1));
+    ctx->h_bcflag.assign(m->bcflag, m->bcflag + m->nnode);
+    return DES_OK;
 }
+
+void des_set_mesh_order(int mode) { g_mesh_order = mode ? 1 : 0; }
 
 void des_destroy(des_ctx *ctx)
 {
@@ -2045,8 +2161,7 @@ int field_opt(des_ctx *ctx, double **slot, const double *src, long n, long off)
         int rc = dalloc_n(ctx, slot, (size_t)ctx->E);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(*slot + off, src, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
-    return DES_OK;
+    return upload_soa(ctx, *slot, src, n, 1, ctx->E, off, ctx->eh.data() + off);
 }
 
 // The fields of des_fields between host AoS arrays and the device SoA state:
@@ -2074,14 +2189,17 @@ int xfer_fields(des_ctx *ctx, des_fields *f, bool up, int model)
     };
     for (auto &t : tab) {
         const long stride = t.elem ? ES : NS, off = t.elem ? eo : no;
-        rc = up ? upload_soa(ctx, t.dev, t.host, t.n, t.k, stride, off)
-                : download_soa(ctx, t.host, t.dev, t.n, t.k, stride, off);
+        const int *perm = (t.elem ? ctx->eh.data() : ctx->nh.data()) + off;
+        rc = up ? upload_soa(ctx, t.dev, t.host, t.n, t.k, stride, off, perm)
+                : download_soa(ctx, t.host, t.dev, t.n, t.k, stride, off, perm);
         if (rc) return rc;
     }
     double *shp_host[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
     for (int d = 0; d < nd; ++d) {
         double *dev = s.shp + (size_t)d * npe * ES;
-        rc = up ? upload_soa(ctx, dev, shp_host[d], E, npe, ES, eo) : download_soa(ctx, shp_host[d], dev, E, npe, ES, eo);
+        const int *perm = ctx->eh.data() + eo;
+        rc = up ? upload_soa(ctx, dev, shp_host[d], E, npe, ES, eo, perm)
+                : download_soa(ctx, shp_host[d], dev, E, npe, ES, eo, perm);
         if (rc) return rc;
     }
     if (up) {
@@ -2138,9 +2256,12 @@ int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_t
     double *d_diag;
     HIPCHK(hipMallocAsync((void **)&d_diag, sizeof(double) * 3 * E, ctx->stream));
     LAUNCH_ND(k_elem_diagnostics, nblocks(E), ctx->s, d_diag, d_diag + E, d_diag + 2 * E);
-    HIPCHK(hipMemcpyAsync(density.data(), d_diag, sizeof(double) * E, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(quality.data(), d_diag + E, sizeof(double) * E, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(viscosity.data(), d_diag + 2 * E, sizeof(double) * E, hipMemcpyDeviceToHost, ctx->stream));
+    {
+        const int *perm = ctx->eh.data();
+        if ((rc = download_soa(ctx, density.data(), d_diag, E, 1, E, 0, perm))) return rc;
+        if ((rc = download_soa(ctx, quality.data(), d_diag + E, E, 1, E, 0, perm))) return rc;
+        if ((rc = download_soa(ctx, viscosity.data(), d_diag + 2 * E, E, 1, E, 0, perm))) return rc;
+    }
     HIPCHK(hipFreeAsync(d_diag, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->s.elemmarkers) {       // most abundant material (output.cxx:168-173)
@@ -2148,7 +2269,7 @@ int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_t
         HIPCHK(hipMemcpy(em.data(), ctx->s.elemmarkers, sizeof(int) * em.size(), hipMemcpyDeviceToHost));
         for (long e = 0; e < E; ++e) {
             const int *a = em.data() + e * ctx->P.nmat;
-            material[e] = (double)(std::max_element(a, a + ctx->P.nmat) - a);
+            material[ctx->eh[e]] = (double)(std::max_element(a, a + ctx->P.nmat) - a);
         }
     }
     DevScalars sc{};
@@ -2215,7 +2336,11 @@ int des_upload_elemmarkers(des_ctx *ctx, const int *counts)
         if ((rc = dalloc_n(ctx, &em, n))) return rc;
         ctx->s.elemmarkers = em;
     }
-    HIPCHK(hipMemcpyAsync(const_cast<int *>(ctx->s.elemmarkers), counts, sizeof(int) * n,
+    const int nmat = ctx->P.nmat;
+    std::vector<int> dev(n);
+    for (long e = 0; e < ctx->E; ++e)
+        for (int j = 0; j < nmat; ++j) dev[(size_t)e * nmat + j] = counts[(size_t)ctx->eh[e] * nmat + j];
+    HIPCHK(hipMemcpyAsync(const_cast<int *>(ctx->s.elemmarkers), dev.data(), sizeof(int) * n,
                           hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
@@ -2402,43 +2527,71 @@ int des_step(des_ctx *ctx)
 
 namespace {
 
-// Timing of the fused passes: an event after each launch on the context's
-// stream; pass k's time is the gap between its event and the previous one.
+// Timing of the fused passes.  arm() before a pass hands its launch a
+// start/stop event pair that the dispatch itself timestamps: the kernel's
+// execution time, as a kernel trace reports it.  arm(true) brackets several
+// launches with events recorded on the stream instead (launch gaps included).
+// mark(k) books the armed pair under pass kind k.
 struct PassTimer {
+    struct Span {
+        hipEvent_t a, b;
+        int kind;
+    };
     des_ctx *ctx;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> kind;
+    std::vector<Span> sp;
+    hipEvent_t a = nullptr, b = nullptr;
+    bool bracket = false;
     bool on;
     explicit PassTimer(des_ctx *c, bool enable) : ctx(c), on(enable) {}
     ~PassTimer()
     {
-        for (auto e : ev) (void)hipEventDestroy(e);
+        if (!on) return;
+        ctx->ev_a = ctx->ev_b = nullptr;
+        for (auto &x : sp) {
+            (void)hipEventDestroy(x.a);
+            (void)hipEventDestroy(x.b);
+        }
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    int arm(bool several = false)
+    {
+        if (!on) return DES_OK;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        bracket = several;
+        if (bracket) {
+            HIPCHK(hipEventRecord(a, ctx->stream));
+        } else {
+            ctx->ev_a = a;
+            ctx->ev_b = b;
+        }
+        return DES_OK;
     }
     int mark(int k)
     {
         if (!on) return DES_OK;
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        ev.push_back(e);
-        kind.push_back(k);
-        HIPCHK(hipEventRecord(e, ctx->stream));
+        if (bracket) HIPCHK(hipEventRecord(b, ctx->stream));
+        ctx->ev_a = ctx->ev_b = nullptr;
+        sp.push_back({a, b, k});
+        a = b = nullptr;
         return DES_OK;
     }
     int collect(double *ms, int *counts)
     {
-        if (!on || ev.empty()) return DES_OK;
-        HIPCHK(hipEventSynchronize(ev.back()));
-        for (size_t i = 1; i < ev.size(); ++i) {
-            if (kind[i] < 0) continue;
+        if (!on || sp.empty()) return DES_OK;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (auto &x : sp) {
             float t = 0;
-            HIPCHK(hipEventElapsedTime(&t, ev[i - 1], ev[i]));
-            ms[kind[i]] += t;
-            counts[kind[i]] += 1;
+            HIPCHK(hipEventElapsedTime(&t, x.a, x.b));
+            ms[x.kind] += t;
+            counts[x.kind] += 1;
         }
         return DES_OK;
     }
 };
 
+#define ARM(several) do { if ((rc = tm.arm(several))) return rc; } while (0)
 #define MARK(k) do { if ((rc = tm.mark(k))) return rc; } while (0)
 
 // the deferred-return list of pass C (3D), allocated on first use
@@ -2490,7 +2643,7 @@ constexpr int PASSC_VARIANTS = 4;
 int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
 {
     dim3 g(nblocks(ctx->E)), b(BLOCK);
-#define PC(NDV, W) hipLaunchKernelGGL((k_pass_c<NDV, W>), g, b, 0, ctx->stream, ctx->s, fl)
+#define PC(NDV, W) hipExtLaunchKernelGGL((k_pass_c<NDV, W>), g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s, fl)
     if (ctx->nd == 3) {
         switch (variant) {
         case 0: PC(3, 1); break;
@@ -2514,7 +2667,8 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
 int launch_pass_c_ret(des_ctx *ctx)
 {
     if (ctx->nd != 3) return DES_OK;
-    hipLaunchKernelGGL(k_pass_c_ret, dim3(std::min(nblocks(ctx->E), 2048)), dim3(BLOCK), 0, ctx->stream, ctx->s);
+    hipExtLaunchKernelGGL(k_pass_c_ret, dim3(std::min(nblocks(ctx->E), 2048)), dim3(BLOCK), 0, ctx->stream,
+                          ctx->ev_a, ctx->ev_b, 0, ctx->s);
     return check_launch(ctx);
 }
 
@@ -2554,11 +2708,14 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     fl.use_stored_mass = (k == 0);
     fl.store_aux = (k == nsteps - 1);
     if (k > 0) swap_volume(ctx);
+    ARM(false);
     LAUNCH_ND(k_pass_a, nblocks(ctx->E), ctx->s, fl);
     MARK(0);
+    ARM(false);
     LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
     MARK(1);
     auto &tune = ctx->tune;
+    ARM(false);
     if (mixed) {
         LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
     } else if (ctx->passc_variant < 0 && tune.size() < 2 * PASSC_VARIANTS && nsteps >= 16) {
@@ -2594,31 +2751,31 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     }
     MARK(2);
     if (!mixed) {
+        ARM(false);
         if ((rc = launch_pass_c_ret(ctx))) return rc;
         MARK(8);
     }
+    ARM(false);
     LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s);
     MARK(3);
+    ARM(false);
     if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
     else LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
     MARK(4);
     if (ctx->nactive) {
+        ARM(false);
         LAUNCH_ND(k_facet_traction, nblocks(ctx->nactive), ctx->s, ctx->active_facets, ctx->nactive);
         MARK(5);
     }
-    LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, fl);
+    ARM(false);
+    FusedFlags ff = fl;
+    ff.step_end = 1;
+    LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, ff);
     MARK(6);
-    if (ctx->nmodels) {
-        hipLaunchKernelGGL(k_step_end_b, dim3(nblocks(ctx->nmodels)), dim3(BLOCK), 0, ctx->stream, ctx->s);
-    } else {
-        const int rotate = (ctx->P.rheol_type & DES_RH_ELASTIC) ? 1 : 0;
-        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, ctx->stream, ctx->s, rotate);
-    }
-    if ((rc = check_launch(ctx))) return rc;
-    MARK(-1);
     ctx->steps++;
     ctx->closing_pending = true;
     if (ctx->steps % 10 == 0) {
+        ARM(true);
         if ((rc = launch_compute_dt(ctx))) return rc;
         MARK(7);
     }
@@ -2637,7 +2794,6 @@ int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
     int rc = fused_begin(ctx, precision);
     if (rc) return rc;
     const bool mixed = precision == DES_MIXED;
-    MARK(-1);
     for (int k = 0; k < nsteps; ++k)
         if ((rc = fused_step(ctx, k, nsteps, mixed, tm))) return rc;
     return fused_end(ctx, mixed);
@@ -2743,12 +2899,17 @@ int build_group(des_batch *b, const std::vector<int> &ids, const des_params *par
     std::vector<double> coord((size_t)N * nd, 0.0);
     std::vector<unsigned> bcflag(N, 0u);
     std::vector<int> markers(nmat > 1 ? (size_t)E * nmat : 0, 0);
+    std::vector<int> eh(E, -1), nh(N, -1);
     bdry.reserve(F);
     felem.reserve(F);
     fface.reserve(F);
     for (size_t k = 0; k < ids.size(); ++k) {
-        const des_mesh &m = meshes[ids[k]];
+        OrderedMesh om;
+        order_mesh(meshes[ids[k]], nd, nmat, g_mesh_order, om);
+        const des_mesh &m = om.m;
         const int eo = g.e_off[k], no = g.n_off[k];
+        std::copy(om.eh.begin(), om.eh.end(), eh.begin() + eo);
+        std::copy(om.nh.begin(), om.nh.end(), nh.begin() + no);
         for (long i = 0; i < (long)m.nelem * npe; ++i) conn[(size_t)eo * npe + i] = m.connectivity[i] + no;
         std::memcpy(coord.data() + (size_t)no * nd, m.coord, sizeof(double) * m.nnode * nd);
         std::memcpy(bcflag.data() + no, m.bcflag, sizeof(unsigned) * m.nnode);
@@ -2782,7 +2943,7 @@ int build_group(des_batch *b, const std::vector<int> &ids, const des_params *par
     sm.markers_per_element = M0.markers_per_element;
     des_ctx *ctx = new des_ctx();
     b->groups.push_back(ctx);
-    int rc = create_impl(ctx, &P0, &sm, &g);
+    int rc = create_impl(ctx, &P0, &sm, &g, eh, nh);
     if (rc) return bcheck(b, ctx, rc);
     for (size_t k = 0; k < ids.size(); ++k) {
         b->group_of[ids[k]] = (int)b->groups.size() - 1;
@@ -3013,9 +3174,9 @@ int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem)
             int bi = 0;
             const int b0 = ctx->e_off[k] / BLOCK, b1 = nblocks(ctx->e_off[k] + ctx->e_cnt[k]);
             for (int blk = b0; blk < b1; ++blk)
-                if (hq[blk] < best) {
+                if (hq[blk] < best || (hq[blk] == best && best < 1 && hi[blk] < bi)) {
                     best = hq[blk];
-                    bi = hi[blk] - ctx->e_off[k];
+                    bi = hi[blk];
                 }
             q[m] = best;
             worst_elem[m] = bi;

@@ -46,8 +46,9 @@ struct DevState {
     const int *conn;            // E x NPE
     const unsigned *bcflag;     // N
     const int *elemmarkers;     // E x nmat, or null
+    const int *ehost, *nhost;   // device element / node -> host index (model-local in a group)
     const int *csr_off;         // N+1
-    const int *csr_ent;         // e*NPE+i, ascending e per node
+    const int *csr_ent;         // e*NPE+i, ascending host element index per node
     const int *ff_off;          // N+1: node -> traction-loaded facets
     const int *ff_ent;          // facet index, ascending
     const int *bf_bdry, *bf_elem, *bf_face;
@@ -61,7 +62,7 @@ struct DevState {
     double *shp;                                  // ND x NPE x E
     const double *elem_visc, *elem_cohesion;      // E or null
     double *ework;                                // E x NPE scalar contributions (temperature)
-    double *fwork;                                // E x NPE x ND force contributions
+    double *fwork;                                // force contributions, ND per CSR slot
     const int *slot;                              // E x NPE: CSR position of entry e*NPE+i
     double *erec;                                 // E x 8 packed records (fused pass A -> B)
     double *escal;                                // 4 x E element scalars

@@ -28,6 +28,7 @@ EXPORTS = (
     "des_batch_get_scalars", "des_batch_init_derived", "des_batch_apply_vbcs",
     "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
     "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
+    "des_set_mesh_order",
 )
 
 PER_FUNCTION = (
@@ -78,6 +79,8 @@ def load():
     lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     lib.des_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.des_set_tuning.argtypes = [vp, C.c_int]
+    lib.des_set_mesh_order.argtypes = [C.c_int]
+    lib.des_set_mesh_order.restype = None
     lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
     lib.des_batch_destroy.argtypes = [vp]
     lib.des_batch_destroy.restype = None
@@ -100,6 +103,12 @@ def load():
     lib.des_adj_release.restype = None
     _lib = lib
     return lib
+
+
+def set_mesh_order(mode):
+    """Device numbering of meshes created from now on in this process:
+    1 Morton order (default), 0 the host's order.  Results do not depend on it."""
+    load().des_set_mesh_order(int(mode))
 
 
 class DesModel:

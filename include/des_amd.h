@@ -149,6 +149,13 @@ typedef struct des_ctx des_ctx;
    them (fields.cxx:10-48). */
 int des_create(const des_params *p, const des_mesh *m, des_ctx **out);
 void des_destroy(des_ctx *ctx);
+/* Device numbering of the meshes given to later des_create / des_batch_create
+   calls (process-wide; no counterpart in the reference): 1 (default) stores
+   elements and nodes in Morton order of their positions, 0 in the host's
+   order.  Every array crossing this interface stays in the host's numbering
+   and every index returned is a host index; each node sums its elements in
+   ascending host order, so results do not depend on the setting. */
+void des_set_mesh_order(int mode);
 const char *des_last_error(const des_ctx *ctx);
 
 int des_upload_fields(des_ctx *ctx, const des_fields *f);

@@ -54,7 +54,7 @@ def _layers(m):
     return em
 
 
-def build(name, nelem=None):
+def build(name, nelem=None, **mesh_kw):
     cfg = dict(BASE)
     kw = {}
     nd = 3 if name.endswith("3") else 2
@@ -138,7 +138,7 @@ def build(name, nelem=None):
         cfg.update({"control.is_quasi_static": False, "control.damping_option": 0})
     else:
         raise ValueError(name)
-    m = M.Model(cfg, nd, nelem_target=ne, **kw)
+    m = M.Model(cfg, nd, nelem_target=ne, **kw, **mesh_kw)
     if perturb:
         _perturb_stress(m, perturb, 11)
     return m

@@ -1,0 +1,110 @@
+"""The device numbering of the mesh (des_set_mesh_order: Morton order of
+positions, or the host's order) is invisible at the interface: both give
+bitwise equal fields, scalars, quality indices, output frames, batched
+models and gradients, because every transfer permutes rows back and every
+node sums its elements in ascending host order."""
+import numpy as np
+import pytest
+
+import scenarios as SC
+from dynearthsol_amd import abi
+from dynearthsol_amd import model as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(m, order, **kw):
+    from dynearthsol_amd.lib import DesModel, set_mesh_order
+    set_mesh_order(order)
+    try:
+        return DesModel(m, **kw)
+    finally:
+        set_mesh_order(1)
+
+
+def _same(a, b):
+    assert a.keys() == b.keys()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2", "mat3", "mat2", "bc3", "bc2", "damp23", "fields3",
+                                  "elastic3", "psevp2"])
+def test_fused_run_is_independent_of_numbering(name):
+    m = SC.build(name)
+    print(name, m.nelem, "elements", flush=True)
+    g0, g1 = _make(m, 0), _make(m, 1)
+    _same(g0.download(), g1.download())
+    g0.run(25)
+    g1.run(25)
+    _same(g0.download(), g1.download())
+    assert g0.scalars() == g1.scalars()
+
+
+@pytest.mark.parametrize("name", ["evp3", "mat2"])
+def test_per_function_steps_are_independent_of_numbering(name):
+    m = SC.build(name)
+    g0, g1 = _make(m, 0), _make(m, 1)
+    g0.step(6)
+    g1.step(6)
+    _same(g0.download(), g1.download())
+
+
+def test_mixed_precision_is_independent_of_numbering():
+    m = SC.build("evp3")
+    g0, g1 = _make(m, 0), _make(m, 1)
+    g0.run(20, abi.DES_MIXED)
+    g1.run(20, abi.DES_MIXED)
+    _same(g0.download(), g1.download())
+
+
+def test_quality_ties_resolve_to_the_first_host_index():
+    """An unjittered box has many elements of equal quality: the index
+    returned is the first in host order, as the reference's loop keeps it."""
+    m = SC.build("evp3", jitter=0.0)
+    g0, g1 = _make(m, 0), _make(m, 1)
+    q0, w0 = g0.worst_quality()
+    q1, w1 = g1.worst_quality()
+    assert (q0, w0) == (q1, w1)
+    # the first strict minimum of the host-order loop
+    from oracle import OracleState
+    qo, wo = OracleState(m).worst_quality()
+    assert wo == w1 and abs(q1 - qo) <= 1e-14 * qo
+
+
+def test_tiny_elements_report_the_first_host_index():
+    m = SC.build("evp3", nelem=3000)
+    nd = m.ndims
+    smallest = m.params.smallest_size * 0.118 * m.params.resolution ** nd
+    for order in (0, 1):
+        g = _make(m, order)
+        vol = g.download(["volume"])["volume"]
+        vol[[1700, 900, 2400]] = smallest * 0.5
+        g.upload(volume=vol)
+        assert g.bad_mesh_quality() == (3, 900)
+
+
+def test_output_frames_are_identical(tmp_path):
+    m = SC.build("mat3")
+    for order in (0, 1):
+        g = _make(m, order)
+        g.run(10)
+        g.write_frame(tmp_path / f"o{order}", 0, run_time=1.0)
+    a = (tmp_path / "o0.save.000000").read_bytes()
+    b = (tmp_path / "o1.save.000000").read_bytes()
+    assert a == b
+
+
+def test_gradient_is_independent_of_numbering():
+    m = M.config_adjoint("plastic", nelem_target=800, log_visc=(19.0, 21.0))
+    SC._perturb_stress(m, 30e6, 11)
+    m.elem_cohesion[:] *= 0.2
+    top = (m.bcflag & 32) != 0
+    cb = np.zeros((m.nnode, 3))
+    cb[top, 2] = np.random.default_rng(0).standard_normal(int(top.sum()))
+    out = []
+    for order in (0, 1):
+        g = _make(m, order)
+        g.adj_forward(12, 5)
+        out.append(g.adj_backward(cb))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
