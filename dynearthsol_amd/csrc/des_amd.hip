@@ -1430,23 +1430,93 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     float pl = s.pl32[e], dpl = s.dpl32[e];
     double dp;
     const float Tm = (float)node_mean<ND>(Tn);
-    bool ok = stress_update_mixed<ND>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p, sd,
-                                      rs, es, re, edot, pl, dpl, dp);
+    bool deferred;
+    MixedRet rec;
+    bool ok = stress_update_mixed<ND, ND == 3>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p,
+                                               sd, rs, es, re, edot, pl, dpl, dp, deferred, rec);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
-    s.pres[e] = p;
 #pragma unroll
-    for (int k = 0; k < NSTR; ++k) {
-        s.sdev[k * E + e] = sd[k];
-        s.strain32[k * E + e] = es[k];
+    for (int k = 0; k < NSTR; ++k) s.strain32[k * E + e] = es[k];
+    if (!deferred) {
+        s.pres[e] = p;
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) s.sdev[k * E + e] = sd[k];
+        s.pl32[e] = pl;
+        s.dpl32[e] = dpl;
+        s.dpressure[e] = dp;
+        s.escal[e] = dp * vol;
     }
-    s.pl32[e] = pl;
-    s.dpl32[e] = dpl;
-    s.dpressure[e] = dp;
-    s.escal[e] = dp * vol;
     if (fl.store_aux) {
 #pragma unroll
         for (int k = 0; k < NSTR; ++k) s.strain_rate[k * E + e] = (double)edot[k];
         s.edvoldt[e] = edv;
+    }
+    if constexpr (ND == 3) {
+        // append the deferred elements, one atomic per wavefront
+        const unsigned long long mask = __ballot(deferred);
+        if (mask) {
+            const int lane = __lane_id();
+            const int leader = __ffsll((long long)mask) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(s.ret_count, __popcll(mask));
+            base = __shfl(base, leader);
+            if (deferred) {
+                const int k = base + __popcll(mask & ((1ull << lane) - 1));
+                s.ret_elem[k] = e;
+                s.ret_mix[k] = rec;
+            }
+        }
+    }
+}
+
+// The Mohr-Coulomb returns that the mixed pass C deferred (3D), completing
+// each element as stress_update_mixed's inline return would.
+__global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
+{
+    const int cnt = *s.ret_count;
+    const long E = s.E;
+    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < cnt; k += gridDim.x * BLOCK) {
+        const int e = s.ret_elem[k];
+        DevState st = s;
+        if (s.eblk_model) bind_model<3>(st, s.eblk_model[e / BLOCK]);
+        const des_params *P = st.P;
+        const MixedRet &r = s.ret_mix[k];
+        const float K = (float)harmonic_mean(st, P->bulk_modulus, e);
+        const float G = (float)harmonic_mean(st, P->shear_modulus, e);
+        float pl = s.pl32[e], dpl = s.dpl32[e];
+        double amc, anphi, anpsi, hardn, ten_max;
+        mat_plastic_props_f(st, e, pl, amc, anphi, anpsi, hardn, ten_max);
+        double s64[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s64[i] = r.s64[i];
+        double depls;
+        mc_return<3>((double)K, (double)G, amc, anphi, anpsi, hardn, ten_max, depls, s64);
+        float sd[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sd[i] = s.sdev[i * E + e];
+        double p;
+        if (P->rheol_type == DES_RH_EVP && (double)r.svII < second_invariant2<3>(s64)) {
+            p = r.pv;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) sd[i] += r.incv[i];
+        } else if (depls == 0) {
+            p = r.pp;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) sd[i] += r.inc[i];
+            dpl = 0;
+        } else {
+            split_stress<3>(s64, p, sd);
+            pl += (float)depls;
+            dpl = (float)depls;
+        }
+        s.pres[e] = p;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) s.sdev[i * E + e] = sd[i];
+        s.pl32[e] = pl;
+        s.dpl32[e] = dpl;
+        const double dp = 3 * (p - r.p_old);
+        s.dpressure[e] = dp;
+        s.escal[e] = dp * s.volume[e];
     }
 }
 
@@ -2622,6 +2692,7 @@ int ensure_mixed(des_ctx *ctx)
     if ((rc = dalloc_n(ctx, &s.strain32, E * nstr))) return rc;
     if ((rc = dalloc_n(ctx, &s.pl32, E))) return rc;
     if ((rc = dalloc_n(ctx, &s.dpl32, E))) return rc;
+    if (ctx->nd == 3 && (rc = dalloc_n(ctx, &s.ret_mix, E))) return rc;
     if (s.elem_visc) {
         float *v;
         if ((rc = dalloc_n(ctx, &v, E))) return rc;
@@ -2664,11 +2735,12 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
 }
 
 // the Mohr-Coulomb returns that pass C deferred (3D)
-int launch_pass_c_ret(des_ctx *ctx)
+int launch_pass_c_ret(des_ctx *ctx, bool mixed = false)
 {
     if (ctx->nd != 3) return DES_OK;
-    hipExtLaunchKernelGGL(k_pass_c_ret, dim3(std::min(nblocks(ctx->E), 2048)), dim3(BLOCK), 0, ctx->stream,
-                          ctx->ev_a, ctx->ev_b, 0, ctx->s);
+    const dim3 g(std::min(nblocks(ctx->E), 2048)), b(BLOCK);
+    if (mixed) hipExtLaunchKernelGGL(k_pass_c_ret_mixed, g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s);
+    else hipExtLaunchKernelGGL(k_pass_c_ret, g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s);
     return check_launch(ctx);
 }
 
@@ -2750,9 +2822,9 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
     }
     MARK(2);
-    if (!mixed) {
+    if (ctx->nd == 3) {
         ARM(false);
-        if ((rc = launch_pass_c_ret(ctx))) return rc;
+        if ((rc = launch_pass_c_ret(ctx, mixed))) return rc;
         MARK(8);
     }
     ARM(false);

@@ -38,6 +38,16 @@ struct DevScalars {
     int pad;
 };
 
+// What a mixed-precision 3D element whose elastic trial is not clearly inside
+// the Mohr-Coulomb surface leaves for the return kernel: the FP64 trial, the
+// pressures of the elastic trial (pp), of the Maxwell stress (pv) and before
+// the update (p_old), and the FP32 deviator increments of both candidates.
+struct MixedRet {
+    double s64[6], pp, pv, p_old;
+    float inc[6], incv[6], svII, pad;
+};
+static_assert(sizeof(MixedRet) == 128, "one 128-byte line per record");
+
 // Every device array of one model.  Node vectors and element tensors are
 // component-major (SoA); connectivity stays AoS as in the reference.
 struct DevState {
@@ -92,6 +102,7 @@ struct DevState {
     // the stress trace before the update (8 doubles)
     int *ret_count, *ret_elem;
     double *ret_val;
+    MixedRet *ret_mix;                            // the mixed-precision schedule's records
     // node records of the fused schedule (NREC doubles per node): x[ND],
     // v[ND], T, aux (dvoldt after pass B, dp_nd after pass D), one cache
     // line per node for the element passes' gathers
@@ -1300,13 +1311,20 @@ template <int ND> __device__ __forceinline__ void split_stress(const double *s, 
 // update_stress for one element in mixed precision (plane strain excluded).
 // rs / re: Jaumann increments of the deviator and the strain from the
 // previous step's rotation, folded into this step's single rounding.
-template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf,
-                                                                     float edvoldt, float Tmean, double vol,
-                                                                     double vol_old, double &p, float *sd,
-                                                                     const float *rs, float *es,
-                                                                     const float *re, float *edot, float &pl,
-                                                                     float &dpl, double &dpressure)
+//
+// DEFER (3D): an elasto-(visco-)plastic element whose elastic trial is not
+// clearly elastic (clearly_elastic3) fills rec and sets deferred; p, sd, pl
+// and dpl are then untouched and the return kernel completes the element.
+// Every other element gets exactly what the inline return gives it, since
+// the return leaves a clearly elastic trial unchanged.
+template <int ND, bool DEFER = false>
+__device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf, float edvoldt,
+                                                    float Tmean, double vol, double vol_old, double &p,
+                                                    float *sd, const float *rs, float *es, const float *re,
+                                                    float *edot, float &pl, float &dpl, double &dpressure,
+                                                    bool &deferred, MixedRet &rec)
 {
+    deferred = false;
     constexpr int NSTR = Dim<ND>::NSTR;
     const des_params *P = st.P;
     const double p_old = p;
@@ -1352,6 +1370,8 @@ template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevS
     case DES_RH_EVP: {
         double pv = p;
         float incv[NSTR];
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) incv[i] = 0;
         float svII = 0;
         if (P->rheol_type == DES_RH_EVP) {
             float eta = mat_visc_f<ND>(st, e, Tmean, edot);
@@ -1370,8 +1390,27 @@ template <int ND> __device__ __forceinline__ bool stress_update_mixed(const DevS
         for (int i = 0; i < ND; ++i) s64[i] = (double)sd[i] + (double)inc[i] + pp;
 #pragma unroll
         for (int i = ND; i < NSTR; ++i) s64[i] = (double)sd[i] + (double)inc[i];
-        double depls;
-        mc_return<ND>((double)K, (double)G, amc, anphi, anpsi, hardn, ten_max, depls, s64);
+        if constexpr (DEFER && ND == 3) {
+            if (!clearly_elastic3(s64, amc, anphi, ten_max)) {
+                deferred = true;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    rec.s64[i] = s64[i];
+                    rec.inc[i] = inc[i];
+                    rec.incv[i] = incv[i];
+                }
+                rec.pp = pp;
+                rec.pv = pv;
+                rec.p_old = p_old;
+                rec.svII = svII;
+                dpressure = 0;
+                return ok;
+            }
+            // clearly elastic: the return would leave s64 as it is, depls = 0
+        }
+        double depls = 0;
+        if constexpr (!(DEFER && ND == 3))
+            mc_return<ND>((double)K, (double)G, amc, anphi, anpsi, hardn, ten_max, depls, s64);
         if (P->rheol_type == DES_RH_EVP && (double)svII < second_invariant2<ND>(s64)) {
             p = pv;
 #pragma unroll
