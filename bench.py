@@ -37,11 +37,14 @@ PASS_BYTES_3D = {
 # mixed precision: FP32 deviator/strain/plastic strain in passes C and E
 PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
 
-# HBM bytes per launch of the dominant kernel from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for
-# a plain 1 GiB copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved);
-# measured on this workload, the command in profiles/.
-TRAFFIC = {}
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
+# dispatch, averaged over 600 dispatches), each scaled by what the counter
+# reads for a plain 1 GiB copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the
+# bytes moved): (2 FETCH + WRITE) * 1024.  Measured on the 3d-fp64 workload,
+# 1019592 tetrahedra; the commands are in profiles/.
+TRAFFIC = {
+    "3d-fp64": {"k_pass_e": 252487228, "k_pass_f": 156298403, "k_pass_b": 310944975},
+}
 
 WORKLOADS = {
     "3d-fp64": ("3d-fp64 extension box 100x100x10 km, EVP Mohr-Coulomb", 1_000_000),
@@ -307,7 +310,8 @@ def main():
                      "launches": tot_n, "timed_window_avg_ms": avg[dom],
                      "bytes_per_launch": be * E + bn * N,
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get(args.workload)},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC.get(args.workload, {}).get(kernel) if nelem == ne_default else None},
         "step_hbm": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},
