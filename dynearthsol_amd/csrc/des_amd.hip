@@ -21,7 +21,6 @@
 #include <cstring>
 #include <string>
 #include <initializer_list>
-#include <tuple>
 #include <vector>
 
 #include "des_amd.h"
@@ -1015,6 +1014,14 @@ struct FusedFlags {
     int use_stored_mass = 0;   // first step after a sync: mass/tmass/volume_n from compute_mass
     int store_aux = 0;         // last step before a sync: keep strain_rate, edvoldt, force
     int step_end = 0;          // pass F also ends the step: time, steps, dt_prev, rotation flag
+    // the differentiated run's step records, written by the pass that has
+    // the value at hand (null in plain runs): pass C (FP64) the stress,
+    // plastic strain and volume_old it starts from, pass B the node records
+    // after it (x and v of the step's start, T, dvoldt), mass and volume_n,
+    // pass D dp_nd, pass F the force
+    double *rec_stress = nullptr, *rec_plstrain = nullptr, *rec_volume_old = nullptr;
+    double *rec_nrec = nullptr, *rec_mass = nullptr, *rec_volume_n = nullptr;
+    double *rec_dpn = nullptr, *rec_force = nullptr;
 };
 
 // the scalar bookkeeping at the end of a step (var.time, var.steps), and the
@@ -1120,14 +1127,27 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
         s.tmass[n] = tm;
     }
     double *rn = s.nrec + (long)NREC * n;
+    double Tnew = 0;
     if (thermal) {
-        const double Tn = temperature_new(s, n, s.T[n], tdot, s.sc->dt, tm);
-        s.T[n] = Tn;
-        rn[NrecSlot<ND>::T] = Tn;
+        Tnew = temperature_new(s, n, s.T[n], tdot, s.sc->dt, tm);
+        s.T[n] = Tnew;
+        rn[NrecSlot<ND>::T] = Tnew;
     }
     const double dvn = dj / vn;
     s.ntmp[n] = dvn;
     rn[NrecSlot<ND>::AUX] = dvn;
+    if (fl.rec_nrec) {
+        double *r = fl.rec_nrec + (long)NREC * n;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            r[NrecSlot<ND>::X + d] = s.coord[d * s.N + n];
+            r[NrecSlot<ND>::V + d] = s.vel[d * s.N + n];
+        }
+        r[NrecSlot<ND>::T] = thermal ? Tnew : rn[NrecSlot<ND>::T];
+        r[NrecSlot<ND>::AUX] = dvn;
+        fl.rec_mass[n] = m;
+        fl.rec_volume_n[n] = vn;
+    }
 }
 
 template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
@@ -1174,8 +1194,11 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     }
     double st[NSTR];
     load_tensor<ND>(s.stress, E, e, st);
+    if (fl.rec_stress) store_tensor<ND>(fl.rec_stress, E, e, st);
     if (rot) jaumann<ND>(st, dtp, w);
     double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
+    if (fl.rec_plstrain) fl.rec_plstrain[e] = pl;
+    if (fl.rec_volume_old) fl.rec_volume_old[e] = s.volume_old[e];
     const double vol = s.volume[e];
     bool ok = true, defer = false;
     double trial[6], old_s = 0;
@@ -1260,7 +1283,7 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
 }
 
 // pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s, FusedFlags fl)
 {
     BIND_NODE(s, n);
     constexpr int NPE = Dim<ND>::NPE;
@@ -1281,6 +1304,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s)
     const double dpn = sum / s.volume_n[n];
     s.ntmp[n] = dpn;
     s.nrec[(long)NREC * n + NrecSlot<ND>::AUX] = dpn;
+    if (fl.rec_dpn) fl.rec_dpn[n] = dpn;
 }
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
@@ -1324,6 +1348,10 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
     if (fl.store_aux) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
+    }
+    if (fl.rec_force) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) fl.rec_force[d * s.N + n] = f[d];
     }
     const double m = s.mass[n];
 #pragma unroll
@@ -2828,7 +2856,7 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         MARK(8);
     }
     ARM(false);
-    LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s);
+    LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s, fl);
     MARK(3);
     ARM(false);
     if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
