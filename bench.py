@@ -145,12 +145,13 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def adjoint_timing(nsteps, window, nelem):
-    """The differentiated run of the visco-elastic adjoint configuration:
-    plain forward, checkpointed forward and reverse sweep, ms per step."""
+def adjoint_timing(nsteps, window, nelem, case="maxwell"):
+    """The differentiated run of the adjoint configuration (visco-elastic or
+    plastic-enabled): plain forward, checkpointed forward and reverse sweep,
+    ms per step."""
     from dynearthsol_amd import model as M
     from dynearthsol_amd.lib import DesModel
-    m = M.config_adjoint("maxwell", nelem_target=nelem)
+    m = M.config_adjoint(case, nelem_target=nelem)
     g = DesModel(m)
     t_plain = timed(g, nsteps, 0)
     g.reset()
@@ -166,7 +167,8 @@ def adjoint_timing(nsteps, window, nelem):
     g.adj_backward(cb)
     t_bwd = time.perf_counter() - t0
     g.close()
-    return {"workload": f"adjoint box 50x50x10 km, {m.nelem} tets, maxwell, log10 viscosity U[22,24]",
+    rheology = "maxwell" if case == "maxwell" else "elasto-visco-plastic"
+    return {"workload": f"adjoint box 50x50x10 km, {m.nelem} tets, {rheology}, log10 viscosity U[22,24]",
             "steps": nsteps, "window": window,
             "forward_ms_per_step": 1e3 * t_plain / nsteps,
             "checkpointed_forward_ms_per_step": 1e3 * t_fwd / nsteps,
@@ -178,12 +180,15 @@ def bench_adjoint(args):
     """The differentiated run (visco-elastic adjoint configuration, 200 k tets,
     2,000 steps, checkpoint every 100): gradient cost over forward cost."""
     adj = adjoint_timing(args.adjoint_steps, 100, args.adjoint_nelem)
+    print(f"# visco-elastic case: {adj['gradient_over_forward']:.3f}x", file=sys.stderr, flush=True)
+    plastic = adjoint_timing(args.adjoint_steps, 100, args.adjoint_nelem, case="plastic")
     out = {
         "metric": METRIC, "value": adj["gradient_over_forward"], "unit": "x forward time",
         "n_gpus": 1, "steps": args.adjoint_steps, "warmup": 0,
         "ms_per_step": adj["checkpointed_forward_ms_per_step"] + adj["backward_ms_per_step"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic", "config": {"workload": adj["workload"], "window": 100}, "adjoint": adj,
+        "adjoint_plastic": plastic,
     }
     print(json.dumps(out), flush=True)
 
