@@ -998,6 +998,45 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_badmesh_partial(Dev
     }
 }
 
+// ---- Output::average_fields (output.cxx:211-251) after a step.  snapshot
+// (the interval's first step, var.steps % average_interval == 1): keep
+// strain, stress and delta_plstrain (coord and the time are kept by the
+// caller and lane 0); otherwise add stress and delta_plstrain to the sums.
+// A rotation the fused driver still owes (rot_pending) is applied to the
+// copies here exactly as the next pass C applies it, so the fields summed are
+// the reference's after rotate_stress.  One model per context.
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_average_fields(DevState s, int snapshot)
+{
+    const int e = xcd_block() * BLOCK + threadIdx.x;
+    if (e >= s.E) return;
+    constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
+    const long E = s.E;
+    if (snapshot && e == 0) s.sc->time0 = s.sc->time;
+    double st[NSTR], es[NSTR];
+    load_tensor<ND>(s.stress, E, e, st);
+    if (snapshot) load_tensor<ND>(s.strain, E, e, es);
+    if (s.sc->rot_pending) {
+        int c[NPE];
+        load_conn<ND>(s, e, c);
+        double x[NPE][ND], v[NPE][ND], Tn[NPE], aux[NPE], sh[ND][NPE], edot[NSTR], w[3];
+        load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
+        shape_fn<ND>(x, elem_volume<ND>(x), sh);
+        strain_rate_spin<ND>(v, sh, edot, w, true);
+        const double dtp = s.sc->dt_prev;
+        jaumann<ND>(st, dtp, w);
+        if (snapshot) jaumann<ND>(es, dtp, w);
+    }
+    if (snapshot) {
+        store_tensor<ND>(s.avg_stress, E, e, st);
+        store_tensor<ND>(s.avg_strain0, E, e, es);
+        s.avg_dpl[e] = s.delta_plstrain[e];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) s.avg_stress[k * E + e] += st[k];
+        s.avg_dpl[e] += s.delta_plstrain[e];
+    }
+}
+
 // ============================================================ fused step
 //
 // Pass A (elements): volume, shape functions, thermal contributions,
@@ -1638,6 +1677,7 @@ struct des_ctx {
     int nactive = 0;
     double *elem0 = nullptr;       // bulkm(0), rho(0)
     int steps = 0;                 // host mirror of var.steps
+    int avg_interval = 0;          // Output::average_interval (0: averaged fields off)
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
     std::vector<int> eh, nh;       // device element / node -> host index (model-local in a group)
     std::vector<int> h_conn;       // host copies for output frames
@@ -2336,9 +2376,16 @@ int des_download_fields(des_ctx *ctx, des_fields *f)
 // "name<TAB>offset" lines, then the raw arrays) with the field set of
 // Output::write (output.cxx:93-179, instantaneous fields), and one line of
 // the .info table (output.cxx:43-76).
-int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_time)
+}  // extern "C"
+
+namespace {
+
+// Output::write (output.cxx:79-208); averaged: with the averaged fields of
+// the current interval (average_interval > 0), as a regular frame writes them
+int write_frame_impl(des_ctx *ctx, const char *modelname, int frame, double run_time, bool averaged)
 {
     if (!ctx || !modelname || frame < 0) return DES_ERR_INPUT;
+    if (averaged && ctx->avg_interval <= 0) return fail(ctx, DES_ERR_STATE, "averaged fields are not enabled");
     int rc = flush_deferred(ctx);
     if (rc) return rc;
     const long N = ctx->N, E = ctx->E;
@@ -2372,6 +2419,27 @@ int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_t
     }
     DevScalars sc{};
     if ((rc = read_scalars(ctx, sc))) return rc;
+    // the averaged variants (output.cxx:83-86, 97-105, 113-119, 121-131, 136-143)
+    double info_dt = sc.dt;
+    std::vector<double> vel_avg, stress_avg;
+    if (averaged) {
+        const double span = sc.time - sc.time0;
+        const double inv_dt = 1.0 / span;
+        info_dt = span / ctx->avg_interval;
+        std::vector<double> c0(N * nd), s0(E * nstr), davg(E);
+        stress_avg.resize(E * nstr);
+        const int *np = ctx->nh.data(), *ep = ctx->eh.data();
+        if ((rc = download_soa(ctx, c0.data(), ctx->s.avg_coord0, N, nd, N, 0, np))) return rc;
+        if ((rc = download_soa(ctx, s0.data(), ctx->s.avg_strain0, E, nstr, E, 0, ep))) return rc;
+        if ((rc = download_soa(ctx, stress_avg.data(), ctx->s.avg_stress, E, nstr, E, 0, ep))) return rc;
+        if ((rc = download_soa(ctx, davg.data(), ctx->s.avg_dpl, E, 1, E, 0, ep))) return rc;
+        vel_avg.resize(N * nd);
+        for (long i = 0; i < N * nd; ++i) vel_avg[i] = (coord[i] - c0[i]) * inv_dt;
+        for (long i = 0; i < E; ++i) dpl[i] = davg[i] * inv_dt;
+        for (long i = 0; i < E * nstr; ++i) sr[i] = (strain[i] - s0[i]) * inv_dt;
+        const double tmp = 1.0 / (ctx->avg_interval + 1);
+        for (double &x : stress_avg) x *= tmp;
+    }
 
     char fname[1024];
     std::snprintf(fname, sizeof fname, "%s.save.%06d", modelname, frame);
@@ -2392,12 +2460,14 @@ int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_t
     put("coordinate", coord.data(), sizeof(double) * coord.size());
     put("connectivity", ctx->h_conn.data(), sizeof(int) * ctx->h_conn.size());
     put("velocity", vel.data(), sizeof(double) * vel.size());
+    if (averaged) put("velocity averaged", vel_avg.data(), sizeof(double) * vel_avg.size());
     put("temperature", T.data(), sizeof(double) * T.size());
     put("plastic strain", pl.data(), sizeof(double) * pl.size());
     put("plastic strain-rate", dpl.data(), sizeof(double) * dpl.size());
     put("strain-rate", sr.data(), sizeof(double) * sr.size());
     put("strain", strain.data(), sizeof(double) * strain.size());
     put("stress", stress.data(), sizeof(double) * stress.size());
+    if (averaged) put("stress averaged", stress_avg.data(), sizeof(double) * stress_avg.size());
     put("density", density.data(), sizeof(double) * E);
     put("mesh quality", quality.data(), sizeof(double) * E);
     put("viscosity", viscosity.data(), sizeof(double) * E);
@@ -2415,12 +2485,61 @@ int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_t
     std::string info = std::string(modelname) + ".info";
     std::FILE *fi = std::fopen(info.c_str(), frame == 0 ? "w" : "a");
     if (!fi) return fail(ctx, DES_ERR_IO, "cannot open " + info);
-    std::fprintf(fi, "%6d\t%10d\t%12.6e\t%12.4e\t%12.6e\t%8d\t%8d\t%8d\n", frame, sc.steps, sc.time, sc.dt,
+    std::fprintf(fi, "%6d\t%10d\t%12.6e\t%12.4e\t%12.6e\t%8d\t%8d\t%8d\n", frame, sc.steps, sc.time, info_dt,
                  run_time, ctx->N, ctx->E, ctx->nbf);
     std::fclose(fi);
     for (double x : coord)      // output.cxx:194-208
         if (!std::isfinite(x)) return fail(ctx, DES_ERR_RUNTIME, "coordinate becomes NaN or Infinity");
     return DES_OK;
+}
+
+// Output::average_fields for the step just completed (ctx->steps is var.steps)
+int average_fields(des_ctx *ctx)
+{
+    const int snapshot = (ctx->steps % ctx->avg_interval == 1) ? 1 : 0;
+    if (snapshot)
+        HIPCHK(hipMemcpyAsync(ctx->s.avg_coord0, ctx->s.coord, sizeof(double) * ctx->N * ctx->nd,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    LAUNCH_ND(k_average_fields, nblocks(ctx->E), ctx->s, snapshot);
+    return DES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_time)
+{
+    return write_frame_impl(ctx, modelname, frame, run_time, false);
+}
+
+int des_write_frame_averaged(des_ctx *ctx, const char *modelname, int frame, double run_time)
+{
+    return write_frame_impl(ctx, modelname, frame, run_time, true);
+}
+
+int des_set_averaging(des_ctx *ctx, int average_interval)
+{
+    if (!ctx || average_interval < 0) return DES_ERR_INPUT;
+    if (ctx->nmodels) return fail(ctx, DES_ERR_INPUT, "averaged fields are per model, not for batched groups");
+    if (average_interval > 0 && !ctx->s.avg_stress) {
+        DevState &s = ctx->s;
+        const size_t E = ctx->E, N = ctx->N;
+        int rc;
+        if ((rc = dalloc_n(ctx, &s.avg_coord0, N * ctx->nd))) return rc;
+        if ((rc = dalloc_n(ctx, &s.avg_strain0, E * ctx->nstr))) return rc;
+        if ((rc = dalloc_n(ctx, &s.avg_stress, E * ctx->nstr))) return rc;
+        if ((rc = dalloc_n(ctx, &s.avg_dpl, E))) return rc;
+    }
+    ctx->avg_interval = average_interval;
+    return DES_OK;
+}
+
+int des_average_fields(des_ctx *ctx)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    if (ctx->avg_interval <= 0) return fail(ctx, DES_ERR_STATE, "averaged fields are not enabled");
+    return average_fields(ctx);
 }
 
 int des_upload_elemmarkers(des_ctx *ctx, const int *counts)
@@ -2781,6 +2900,8 @@ int fused_begin(des_ctx *ctx, int precision)
         const des_params &P = ctx->nmodels ? ctx->hP[k] : ctx->P;
         if (precision == DES_MIXED && P.is_plane_strain)
             return fail(ctx, DES_ERR_INPUT, "mixed precision does not cover plane strain");
+        if (precision == DES_MIXED && ctx->avg_interval > 0)
+            return fail(ctx, DES_ERR_INPUT, "averaged fields are accumulated by FP64 runs");
     }
     if (ctx->nmodels) {
         // per-element viscosity / cohesion are read for every model or none
@@ -2879,6 +3000,7 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         if ((rc = launch_compute_dt(ctx))) return rc;
         MARK(7);
     }
+    if (ctx->avg_interval > 0 && (rc = average_fields(ctx))) return rc;   // dynearthsol.cxx:363-364
     return DES_OK;
 }
 

@@ -36,6 +36,7 @@ struct DevScalars {
     int err;           // first device-detected error (des_status)
     int rot_pending;   // previous step's rotate_stress still to apply
     int pad;
+    double time0;      // Output::time0: var.time at the averaging interval's first step
 };
 
 // What a mixed-precision 3D element whose elastic trial is not clearly inside
@@ -107,6 +108,10 @@ struct DevState {
     // v[ND], T, aux (dvoldt after pass B, dp_nd after pass D), one cache
     // line per node for the element passes' gathers
     double *nrec;
+    // Output's averaged fields (output.cxx:211-251), allocated when averaging
+    // is on: coord and strain at the interval's first step, the running sums
+    // of stress and delta_plstrain
+    double *avg_coord0, *avg_strain0, *avg_stress, *avg_dpl;
 };
 
 constexpr int NREC = 8;

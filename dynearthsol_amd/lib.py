@@ -28,7 +28,7 @@ EXPORTS = (
     "des_batch_get_scalars", "des_batch_init_derived", "des_batch_apply_vbcs",
     "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
     "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
-    "des_set_mesh_order",
+    "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
 )
 
 PER_FUNCTION = (
@@ -77,6 +77,9 @@ def load():
     lib.des_bad_mesh_quality.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.des_upload_elemmarkers.argtypes = [vp, C.POINTER(C.c_int)]
     lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
+    lib.des_write_frame_averaged.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
+    lib.des_set_averaging.argtypes = [vp, C.c_int]
+    lib.des_average_fields.argtypes = [vp]
     lib.des_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.des_set_tuning.argtypes = [vp, C.c_int]
     lib.des_set_mesh_order.argtypes = [C.c_int]
@@ -208,6 +211,18 @@ class DesModel:
     def write_frame(self, modelname, frame, run_time=0.0):
         """<modelname>.save.<frame> and a line of <modelname>.info (output.cxx:79-208)."""
         self._check(self.lib.des_write_frame(self.ctx, str(modelname).encode(), frame, run_time))
+
+    def write_frame_averaged(self, modelname, frame, run_time=0.0):
+        """A regular frame with the averaged fields (output.cxx:79-208)."""
+        self._check(self.lib.des_write_frame_averaged(self.ctx, str(modelname).encode(), frame, run_time))
+
+    def set_averaging(self, average_interval):
+        """Output::average_fields after every step of des_run (0: off)."""
+        self._check(self.lib.des_set_averaging(self.ctx, average_interval))
+
+    def average_fields(self):
+        """Output::average_fields for the step just completed (per-function drivers)."""
+        self._check(self.lib.des_average_fields(self.ctx))
 
     def upload_elemmarkers(self, counts):
         counts = np.ascontiguousarray(counts, dtype=np.int32)

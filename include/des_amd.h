@@ -169,6 +169,19 @@ int des_download_fields(des_ctx *ctx, des_fields *f);
    (created at frame 0).  Returns 11 when a coordinate is NaN or infinite. */
 int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_time);
 
+/* Averaged fields (Output::average_fields, output.cxx:211-251), on the
+   device.  des_set_averaging: average_interval = the reference's
+   mesh.quality_check_step_interval when sim.is_outputting_averaged_fields is
+   on, 0 to turn it off.  des_run then accumulates after every step (FP64
+   runs); a per-function driver calls des_average_fields after each step, as
+   the main loop does (dynearthsol.cxx:363-364).  des_write_frame_averaged is
+   Output::write (output.cxx:79-208) of a regular frame with them: "velocity
+   averaged", "stress averaged", strain-rate and plastic strain-rate averaged
+   over the interval, and the interval's mean dt in the .info line. */
+int des_set_averaging(des_ctx *ctx, int average_interval);
+int des_average_fields(des_ctx *ctx);
+int des_write_frame_averaged(des_ctx *ctx, const char *modelname, int frame, double run_time);
+
 /* Marker counts per element and material (var.elemmarkers, nelem x nmat),
    e.g. after the host's phase changes (phasechanges.cxx:232-258). */
 int des_upload_elemmarkers(des_ctx *ctx, const int *counts);

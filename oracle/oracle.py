@@ -135,6 +135,35 @@ def adjoint_lib():
     return lib
 
 
+class AveragedFields:
+    """Output::average_fields (output.cxx:211-251) over an OracleState stepped
+    one step at a time, and the averaged arrays Output::write makes of them
+    (output.cxx:83-86, 97-105, 113-131, 136-143).  Test infrastructure."""
+
+    def __init__(self, interval):
+        self.interval = interval
+
+    def after_step(self, o):
+        if o.S.steps % self.interval == 1:     # the interval's first step
+            self.time0 = o.S.time
+            self.coord0 = o.f["coord"].copy()
+            self.strain0 = o.f["strain"].copy()
+            self.stress_avg = o.f["stress"].copy()
+            self.dpl_avg = o.f["delta_plstrain"].copy()
+        else:
+            self.stress_avg += o.f["stress"]
+            self.dpl_avg += o.f["delta_plstrain"]
+
+    def frame(self, o):
+        """The arrays of an averaged frame, and the .info dt."""
+        inv_dt = 1.0 / (o.S.time - self.time0)
+        return {"velocity averaged": (o.f["coord"] - self.coord0) * inv_dt,
+                "plastic strain-rate": self.dpl_avg * inv_dt,
+                "strain-rate": (o.f["strain"] - self.strain0) * inv_dt,
+                "stress averaged": self.stress_avg * (1.0 / (self.interval + 1)),
+                "dt": (o.S.time - self.time0) / self.interval}
+
+
 class OracleState:
     """The reference's Variables as numpy arrays plus the C view of them.
 
