@@ -43,7 +43,7 @@ PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
 # bytes moved): (2 FETCH + WRITE) * 1024.  Measured on the 3d-fp64 workload,
 # 1019592 tetrahedra; the commands are in profiles/.
 TRAFFIC = {
-    "3d-fp64": {"k_pass_e": 252311023, "k_pass_f": 156284341, "k_pass_b": 311045818},
+    "3d-fp64": {"k_pass_e": 250240690, "k_pass_f": 154780961, "k_pass_b": 304984477},
 }
 
 WORKLOADS = {

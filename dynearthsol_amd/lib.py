@@ -110,7 +110,7 @@ def load():
 
 def set_mesh_order(mode):
     """Device numbering of meshes created from now on in this process:
-    1 Morton order (default), 0 the host's order.  Results do not depend on it."""
+    1 Hilbert-curve order (default), 0 the host's order.  Results do not depend on it."""
     load().des_set_mesh_order(int(mode))
 
 

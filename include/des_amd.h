@@ -151,7 +151,7 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out);
 void des_destroy(des_ctx *ctx);
 /* Device numbering of the meshes given to later des_create / des_batch_create
    calls (process-wide; no counterpart in the reference): 1 (default) stores
-   elements and nodes in Morton order of their positions, 0 in the host's
+   elements and nodes in Hilbert-curve order of their positions, 0 in the host's
    order.  Every array crossing this interface stays in the host's numbering
    and every index returned is a host index; each node sums its elements in
    ascending host order, so results do not depend on the setting. */

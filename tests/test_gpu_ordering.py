@@ -1,4 +1,4 @@
-"""The device numbering of the mesh (des_set_mesh_order: Morton order of
+"""The device numbering of the mesh (des_set_mesh_order: Hilbert-curve order of
 positions, or the host's order) is invisible at the interface: both give
 bitwise equal fields, scalars, quality indices, output frames, batched
 models and gradients, because every transfer permutes rows back and every
