@@ -2904,11 +2904,22 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
     return check_launch(ctx);
 }
 
-// the Mohr-Coulomb returns that pass C deferred (3D)
+// whether any model of the context has a plastic rheology (only those defer
+// returns to the list)
+bool any_plastic(const des_ctx *ctx)
+{
+    if (!ctx->nmodels) return (ctx->P.rheol_type & DES_RH_PLASTIC) != 0;
+    for (const des_params &P : ctx->hP)
+        if (P.rheol_type & DES_RH_PLASTIC) return true;
+    return false;
+}
+
+// the Mohr-Coulomb returns that pass C deferred (3D); the list is short, so a
+// small grid walks it
 int launch_pass_c_ret(des_ctx *ctx, bool mixed = false)
 {
-    if (ctx->nd != 3) return DES_OK;
-    const dim3 g(std::min(nblocks(ctx->E), 2048)), b(BLOCK);
+    if (ctx->nd != 3 || !any_plastic(ctx)) return DES_OK;
+    const dim3 g(std::min(nblocks(ctx->E), 512)), b(BLOCK);
     if (mixed) hipExtLaunchKernelGGL(k_pass_c_ret_mixed, g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s);
     else hipExtLaunchKernelGGL(k_pass_c_ret, g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s);
     return check_launch(ctx);
@@ -2994,7 +3005,7 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
     }
     MARK(2);
-    if (ctx->nd == 3) {
+    if (ctx->nd == 3 && any_plastic(ctx)) {
         ARM(false);
         if ((rc = launch_pass_c_ret(ctx, mixed))) return rc;
         MARK(8);
