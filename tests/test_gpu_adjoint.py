@@ -81,6 +81,40 @@ def test_gradient_matches_finite_differences():
         assert abs(vb[e] - fd) <= 1e-5 * abs(fd)
 
 
+def test_gradient_matches_finite_differences_on_random_elements():
+    """64 elements drawn at random (seed 5), as the adjoint configuration's
+    check prescribes: central differences (h = 1e-2 in log viscosity) of the
+    HIP forward, maxwell, no damping, 20 steps, loss on the displacement.
+    Most random elements move the loss by little more than the FP64
+    forward's own rounding, so the bound is 1e-3 relative plus 1e-6 of the
+    largest gradient; the 1e-5 relative check is on the largest gradients
+    (test_gradient_matches_finite_differences)."""
+    from dynearthsol_amd.lib import DesModel
+    m, cb = _case("maxwell", nelem=2000, damping=0)
+    g, vb, _ = _gpu_grad(m, cb, 20, 8)
+    x0 = m.coord.copy()
+    h = 1e-2
+    scale = np.max(np.abs(vb * m.elem_visc))
+    picks = np.random.default_rng(5).choice(m.nelem, 64, replace=False)
+    worst = 0.0
+    for e in picks:
+        vals = []
+        for sgn in (1, -1):
+            a = m.elem_visc.copy()
+            a[e] *= math.exp(sgn * h)
+            gg = DesModel(m)
+            gg.set_elem_params(elem_visc=a)
+            gg.run(20)
+            vals.append(float(np.sum(cb * (gg.download(["coord"])["coord"] - x0))))
+            gg.close()
+        fd = (vals[0] - vals[1]) / (2 * h)          # d loss / d log(visc)
+        ad = vb[e] * m.elem_visc[e]
+        bound = 1e-3 * abs(fd) + 1e-6 * scale
+        worst = max(worst, abs(ad - fd) / bound)
+        assert abs(ad - fd) <= bound, (e, ad, fd)
+    print(f"64 random elements: worst difference {worst:.2f} of the bound")
+
+
 def test_window_size_does_not_change_the_gradient():
     """Recomputation from checkpoints reproduces the forward exactly: the
     gradient is bitwise the same for any window."""
