@@ -200,7 +200,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d-fp64")
     ap.add_argument("--nelem", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
