@@ -147,8 +147,8 @@ def bench_ensemble(args):
 
 def adjoint_timing(nsteps, window, nelem, case="maxwell"):
     """The differentiated run of the adjoint configuration (visco-elastic or
-    plastic-enabled): plain forward, checkpointed forward and reverse sweep,
-    ms per step."""
+    plastic-enabled): plain forward, checkpointed (window > 0) or taped
+    (window 0) forward and reverse sweep, ms per step."""
     from dynearthsol_amd import model as M
     from dynearthsol_amd.lib import DesModel
     m = M.config_adjoint(case, nelem_target=nelem)
@@ -182,13 +182,15 @@ def bench_adjoint(args):
     adj = adjoint_timing(args.adjoint_steps, 100, args.adjoint_nelem)
     print(f"# visco-elastic case: {adj['gradient_over_forward']:.3f}x", file=sys.stderr, flush=True)
     plastic = adjoint_timing(args.adjoint_steps, 100, args.adjoint_nelem, case="plastic")
+    taped = adjoint_timing(args.adjoint_steps, 0, args.adjoint_nelem)
+    print(f"# taped visco-elastic case: {taped['gradient_over_forward']:.3f}x", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": adj["gradient_over_forward"], "unit": "x forward time",
         "n_gpus": 1, "steps": args.adjoint_steps, "warmup": 0,
         "ms_per_step": adj["checkpointed_forward_ms_per_step"] + adj["backward_ms_per_step"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic", "config": {"workload": adj["workload"], "window": 100}, "adjoint": adj,
-        "adjoint_plastic": plastic,
+        "adjoint_plastic": plastic, "adjoint_taped": taped,
     }
     print(json.dumps(out), flush=True)
 

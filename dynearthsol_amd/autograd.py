@@ -10,7 +10,8 @@ The forward restarts the model from its initial state with the given
 per-element parameters, runs nsteps fused FP64 steps on the device with a
 checkpoint every `window` steps, and returns the vertical displacement of the
 top-surface nodes (bcflag & BOUNDZ1).  The backward recomputes each window
-from its checkpoint and runs the hand-written reverse kernels.  Torch tensors
+from its checkpoint (window=0: keeps every step's records from the forward
+and recomputes nothing) and runs the hand-written reverse kernels.  Torch tensors
 carry the parameters and gradients; the model state stays in the library's
 device buffers.
 """

@@ -1695,6 +1695,11 @@ struct des_ctx {
     std::vector<void *> adj_allocs;
     std::vector<AdjCkpt> adj_ckpt;
     int adj_nsteps = 0, adj_window = 0;
+    // taped differentiated run (window 0): every step's records kept from
+    // the forward, 8 buffers per step (NR, S, P, VO, F, m, vn, dpn), and each
+    // step's deferred-rotation flag
+    std::vector<double *> adj_tape;
+    std::vector<int> adj_tape_rot;
 };
 
 // host description of a batched group's concatenated mesh

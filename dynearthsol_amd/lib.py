@@ -249,7 +249,8 @@ class DesModel:
 
     def adj_forward(self, nsteps, window=100):
         """nsteps steps of the differentiated run, checkpointing every window
-        steps (des_adj_forward)."""
+        steps, or keeping every step's records when window is 0
+        (des_adj_forward)."""
         self._check(self.lib.des_adj_forward(self.ctx, nsteps, window))
 
     def adj_backward(self, coord_bar):

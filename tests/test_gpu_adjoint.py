@@ -126,6 +126,30 @@ def test_window_size_does_not_change_the_gradient():
     assert np.array_equal(v1, v3) and np.array_equal(h1, h3)
 
 
+def test_taped_run_matches_checkpointed_run():
+    """window=0 keeps every step's records from the forward: the same
+    gradient bitwise, the same final state, and a second backward over the
+    kept tape gives the same again."""
+    m, cb = _case("plastic", perturb=30e6, coh_scale=0.2)
+    gw, v1, h1 = _gpu_grad(m, cb, 12, 5)
+    gt, v2, h2 = _gpu_grad(m, cb, 12, 0)
+    assert np.array_equal(v1, v2) and np.array_equal(h1, h2)
+    fw = gw.download(["coord", "vel", "stress"])
+    ft = gt.download(["coord", "vel", "stress"])
+    for k in fw:
+        assert np.array_equal(fw[k], ft[k]), k
+    v3, h3 = gt.adj_backward(cb)
+    assert np.array_equal(v2, v3) and np.array_equal(h2, h3)
+
+
+def test_oversized_tape_is_refused():
+    from dynearthsol_amd.lib import DesError, DesModel
+    m, _ = _case("maxwell")
+    g = DesModel(m)
+    with pytest.raises(DesError):
+        g.adj_forward(2_000_000_000, 0)
+
+
 def test_torch_autograd_binding():
     import torch
     from dynearthsol_amd.autograd import SurfaceTopography
