@@ -74,6 +74,47 @@ __device__ __forceinline__ int xcd_block()
         return;                                        \
     }
 
+// The element passes stage their block's node records in LDS: the block's
+// distinct nodes (a compact patch of the curve-ordered mesh) are loaded once,
+// four lanes per 64-byte record, instead of four scattered records per
+// element.  Every lane of the block calls this before any lane returns.
+__device__ __forceinline__ bool stage_block(const DevState &s, int blk, double2 *lds)
+{
+    if (!s.stage) return false;
+    const int2 st = s.stage[blk];
+    if (st.y < 0) return false;
+    const double2 *src = reinterpret_cast<const double2 *>(s.nrec);
+    for (int k = threadIdx.x; k < (NREC / 2) * st.y; k += BLOCK)
+        lds[k] = src[(long)s.stage_node[st.x + k / (NREC / 2)] * (NREC / 2) + k % (NREC / 2)];
+    __syncthreads();
+    return true;
+}
+
+// the records of element e's nodes, from the staged block or from nrec
+template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState &s, int e, bool staged,
+                                                                const double2 *lds,
+                                                                double x[Dim<ND>::NPE][ND],
+                                                                double v[Dim<ND>::NPE][ND],
+                                                                double T[Dim<ND>::NPE],
+                                                                double aux[Dim<ND>::NPE])
+{
+    if (staged) {
+        const unsigned li = s.stage_lidx[e];
+#pragma unroll
+        for (int i = 0; i < Dim<ND>::NPE; ++i)
+            nrec_unpack<ND>(lds + (NREC / 2) * ((li >> (8 * i)) & 255u), x[i], v[i], T[i], aux[i]);
+    } else {
+        int c[Dim<ND>::NPE];
+        load_conn<ND>(s, e, c);
+        load_nrec<ND>(s.nrec, c, x, v, T, aux);
+    }
+}
+
+#define BIND_ELEM_STAGED(s, e, lds, staged)                \
+    __shared__ double2 lds[STAGE_MAX * (NREC / 2)];        \
+    const bool staged = stage_block(s, xcd_block(), lds);  \
+    BIND_ELEM(s, e)
+
 // parameters, scalars and boundary geometry of batched model m
 template <int ND> __device__ __forceinline__ void bind_model(DevState &s, int m)
 {
@@ -1081,13 +1122,11 @@ constexpr int REC_DJ = 4, REC_M = 5, REC_TM = 6, REC_V = 7;
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
 {
-    BIND_ELEM(s, e);
+    BIND_ELEM_STAGED(s, e, lds, staged);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const des_params *P = s.P;
-    int c[NPE];
-    load_conn<ND>(s, e, c);
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
-    load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
+    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, aux);
     double vol = elem_volume<ND>(x);
     shape_fn<ND>(x, vol, sh);
     s.volume[e] = vol;
@@ -1111,51 +1150,67 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, 
     for (int k = 0; k < REC / 2; ++k) out[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
 }
 
+// value of lane k (0..3) of this lane's quad
+template <int K> __device__ __forceinline__ int quad_bcast(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, K * 0x55, 0xf, 0xf, false);
+}
+template <int K> __device__ __forceinline__ double quad_bcast(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = quad_bcast<K>((int)(b & 0xffffffff)), hi = quad_bcast<K>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0 && s.ret_count) *s.ret_count = 0;   // pass C appends
-    BIND_NODE(s, n);
+    // four lanes (a quad) per node: the quad loads an element's 64-byte
+    // record with one 16-byte load per lane and shares it through DPP; every
+    // lane of the quad sums the same values in the same (ascending element)
+    // order, and lane 0 finishes the node
+    const int blk_ = xcd_block();
+    const int t = blk_ * BLOCK + threadIdx.x, n = t >> 2, q = t & 3;
+    if (s.nblk_model) {
+        const int nb = blk_ >> 2;
+        if (n >= s.nblk_lim[nb]) return;
+        bind_model<ND>(s, s.nblk_model[nb]);
+    } else if (n >= s.N) {
+        return;
+    }
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
     double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
     const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
-    // CSR gather in ascending element order, four entries in flight at a time
-    auto entry = [&](int ent, double ti, double2 djm, double2 tmv) {
-        (void)ent;
-        if (thermal) tdot += ti;
-        dj += djm.x;
-        if (!fl.use_stored_mass) {
-            vn += tmv.y;
-            m += djm.y;
-            if (thermal) tm += tmv.x;
-        }
-    };
-    auto load = [&](int ent, double &ti, double2 &djm, double2 &tmv) {
-        const int e = ent / NPE, i = ent - e * NPE;
-        const double *r = s.erec + (long)e * REC;
-        djm = *reinterpret_cast<const double2 *>(r + REC_DJ);
-        tmv = *reinterpret_cast<const double2 *>(r + REC_TM);
-        ti = thermal ? r[i] : 0.0;
-    };
-    int p = p0;
-    for (; p + 4 <= p1; p += 4) {
+    for (int p = p0; p < p1; p += 4) {
+        const int mine = p + q < p1 ? s.csr_ent[p + q] : -1;
         int en[4];
-        double ti[4];
-        double2 djm[4], tmv[4];
+        en[0] = quad_bcast<0>(mine);
+        en[1] = quad_bcast<1>(mine);
+        en[2] = quad_bcast<2>(mine);
+        en[3] = quad_bcast<3>(mine);
+        double2 rq[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) en[j] = s.csr_ent[p + j];
+        for (int j = 0; j < 4; ++j)
+            if (en[j] >= 0) rq[j] = reinterpret_cast<const double2 *>(s.erec + (long)(en[j] / NPE) * REC)[q];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) load(en[j], ti[j], djm[j], tmv[j]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) entry(en[j], ti[j], djm[j], tmv[j]);
+        for (int j = 0; j < 4; ++j) {
+            if (en[j] < 0) break;
+            const int i = en[j] % NPE;
+            const double sel = (i & 1) ? rq[j].y : rq[j].x;
+            const double c0 = quad_bcast<0>(sel), c1 = quad_bcast<1>(sel);
+            const double djv = quad_bcast<2>(rq[j].x), mv = quad_bcast<2>(rq[j].y);
+            const double tmv = quad_bcast<3>(rq[j].x), vv = quad_bcast<3>(rq[j].y);
+            if (thermal) tdot += (i >> 1) ? c1 : c0;
+            dj += djv;
+            if (!fl.use_stored_mass) {
+                vn += vv;
+                m += mv;
+                if (thermal) tm += tmv;
+            }
+        }
     }
-    for (; p < p1; ++p) {
-        double ti;
-        double2 djm, tmv;
-        const int en = s.csr_ent[p];
-        load(en, ti, djm, tmv);
-        entry(en, ti, djm, tmv);
-    }
+    if (q != 0) return;
     if (fl.use_stored_mass) {
         vn = s.volume_n[n];
         m = s.mass[n];
@@ -1190,16 +1245,14 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
 }
 
 template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
-                                                                          int e)
+                                                                          int e, bool staged, const double2 *lds)
 {
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
-    int c[NPE];
-    load_conn<ND>(s, e, c);
     double sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
     {
         double x[NPE][ND];
-        load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
+        load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, aux);
         shape_fn<ND>(x, s.volume[e], sh);
     }
     double w[3];
@@ -1278,8 +1331,10 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
 
 template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
+    // pass C is long enough per element that waves do better starting on
+    // their own gathers than waiting at a staging barrier
     BIND_ELEM(s, e);
-    pass_c_body<ND, true>(s, fl, e);
+    pass_c_body<ND, true>(s, fl, e, false, nullptr);
 }
 
 // Pass C's Mohr-Coulomb returns (rheology.cxx:151-299, 594-635) of the 3D
@@ -1348,13 +1403,11 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s, 
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
 {
-    BIND_ELEM(s, e);
+    BIND_ELEM_STAGED(s, e, lds, staged);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
-    int c[NPE];
-    load_conn<ND>(s, e, c);
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
-    load_nrec<ND>(s.nrec, c, x, v, Tn, dpn);
+    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn);
     double st[NSTR];
     load_tensor<ND>(s.stress, E, e, st);
     {   // NMD_stress's element side (geometry.cxx:221-235), dp_nd from the records
@@ -1454,10 +1507,8 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     BIND_ELEM(s, e);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
-    int c[NPE];
-    load_conn<ND>(s, e, c);
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
-    load_nrec<ND>(s.nrec, c, x, v, Tn, aux);
+    load_elem_nodes<ND>(s, e, false, nullptr, x, v, Tn, aux);
     const double vol = s.volume[e];
     shape_fn<ND>(x, vol, sh);
     // velocities relative to node 0 (sum_i dN_i = 0): no cancellation of the
@@ -1589,13 +1640,11 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
 {
-    BIND_ELEM(s, e);
+    BIND_ELEM_STAGED(s, e, lds, staged);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
-    int c[NPE];
-    load_conn<ND>(s, e, c);
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
-    load_nrec<ND>(s.nrec, c, x, v, Tn, dpn);
+    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn);
     const double dp_el = node_mean<ND>(dpn);
     double p = s.pres[e] + (-s.dpressure[e] + dp_el) / ND;   // NMD_stress on the diagonal mean
     s.pres[e] = p;
@@ -2103,6 +2152,51 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
     H2D(d_ent, ent.data(), ent.size());
     s.csr_off = d_off;
     s.csr_ent = d_ent;
+    {
+        // per element block: its distinct nodes, ascending, and each element's
+        // nodes as positions in that list (blocks above STAGE_MAX: count -1)
+        const int neb = nblocks(E);
+        std::vector<int2> stage(neb);
+        std::vector<int> snode, u;
+        std::vector<unsigned> lidx(E, 0);
+        snode.reserve((size_t)E);
+        for (int b = 0; b < neb; ++b) {
+            const int e0 = b * BLOCK, e1 = std::min(E, e0 + BLOCK);
+            u.clear();
+            for (int e = e0; e < e1; ++e)
+                if (live[e])
+                    for (int i = 0; i < npe; ++i) u.push_back(m->connectivity[(size_t)e * npe + i]);
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            if ((int)u.size() > STAGE_MAX) {
+                stage[b] = make_int2((int)snode.size(), -1);
+                continue;
+            }
+            stage[b] = make_int2((int)snode.size(), (int)u.size());
+            for (int e = e0; e < e1; ++e)
+                if (live[e])
+                    for (int i = 0; i < npe; ++i) {
+                        const int k = (int)(std::lower_bound(u.begin(), u.end(),
+                                                             m->connectivity[(size_t)e * npe + i]) - u.begin());
+                        lidx[e] |= (unsigned)k << (8 * i);
+                    }
+            snode.insert(snode.end(), u.begin(), u.end());
+        }
+        if (snode.empty()) snode.push_back(0);
+        int2 *d_stage;
+        int *d_snode;
+        unsigned *d_lidx;
+        ALLOC(d_stage, neb);
+        ALLOC(d_snode, snode.size());
+        ALLOC(d_lidx, E);
+        H2D(d_stage, stage.data(), neb);
+        H2D(d_snode, snode.data(), snode.size());
+        H2D(d_lidx, lidx.data(), E);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        s.stage = d_stage;
+        s.stage_node = d_snode;
+        s.stage_lidx = d_lidx;
+    }
     {   // CSR position of every entry, for the node-major force contributions
         std::vector<int> slot((size_t)E * npe, 0);
         for (int q = 0; q < off[N]; ++q) slot[ent[q]] = q;
@@ -2972,7 +3066,7 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     LAUNCH_ND(k_pass_a, nblocks(ctx->E), ctx->s, fl);
     MARK(0);
     ARM(false);
-    LAUNCH_ND(k_pass_b, nblocks(ctx->N), ctx->s, fl);
+    LAUNCH_ND(k_pass_b, nblocks(4L * ctx->N), ctx->s, fl);
     MARK(1);
     auto &tune = ctx->tune;
     ARM(false);

@@ -108,6 +108,13 @@ struct DevState {
     // v[ND], T, aux (dvoldt after pass B, dp_nd after pass D), one cache
     // line per node for the element passes' gathers
     double *nrec;
+    // per block of BLOCK elements (the element passes' workgroups): {offset,
+    // count} of its distinct nodes in stage_node (count -1: more than
+    // STAGE_MAX, the block gathers from nrec directly), and per element its
+    // nodes' positions in that list, 8 bits each
+    const int2 *stage;
+    const int *stage_node;
+    const unsigned *stage_lidx;
     // Output's averaged fields (output.cxx:211-251), allocated when averaging
     // is on: coord and strain at the interval's first step, the running sums
     // of stress and delta_plstrain
@@ -115,6 +122,7 @@ struct DevState {
 };
 
 constexpr int NREC = 8;
+constexpr int STAGE_MAX = 256;   // node records an element block stages in LDS
 
 // slots of the node record
 template <int ND> struct NrecSlot {
@@ -122,26 +130,30 @@ template <int ND> struct NrecSlot {
 };
 
 // the records of an element's nodes: coordinates, velocities, T and aux
+template <int ND> __device__ __forceinline__ void nrec_unpack(const double2 *r, double x[ND], double v[ND],
+                                                            double &T, double &aux)
+{
+    if constexpr (ND == 3) {
+        const double2 a = r[0], b = r[1], d = r[2], f = r[3];
+        x[0] = a.x; x[1] = a.y; x[2] = b.x;
+        v[0] = b.y; v[1] = d.x; v[2] = d.y;
+        T = f.x; aux = f.y;
+    } else {
+        const double2 a = r[0], b = r[1], d = r[2];
+        x[0] = a.x; x[1] = a.y;
+        v[0] = b.x; v[1] = b.y;
+        T = d.x; aux = d.y;
+    }
+}
+
 template <int ND> __device__ __forceinline__ void load_nrec(const double *nrec, const int *c,
                                                           double x[Dim<ND>::NPE][ND],
                                                           double v[Dim<ND>::NPE][ND],
                                                           double T[Dim<ND>::NPE], double aux[Dim<ND>::NPE])
 {
 #pragma unroll
-    for (int i = 0; i < Dim<ND>::NPE; ++i) {
-        const double2 *r = reinterpret_cast<const double2 *>(nrec + (long)NREC * c[i]);
-        if constexpr (ND == 3) {
-            const double2 a = r[0], b = r[1], d = r[2], f = r[3];
-            x[i][0] = a.x; x[i][1] = a.y; x[i][2] = b.x;
-            v[i][0] = b.y; v[i][1] = d.x; v[i][2] = d.y;
-            T[i] = f.x; aux[i] = f.y;
-        } else {
-            const double2 a = r[0], b = r[1], d = r[2];
-            x[i][0] = a.x; x[i][1] = a.y;
-            v[i][0] = b.x; v[i][1] = b.y;
-            T[i] = d.x; aux[i] = d.y;
-        }
-    }
+    for (int i = 0; i < Dim<ND>::NPE; ++i)
+        nrec_unpack<ND>(reinterpret_cast<const double2 *>(nrec + (long)NREC * c[i]), x[i], v[i], T[i], aux[i]);
 }
 
 // mean over an element's nodes, in node order (elem_mean's summation)
