@@ -41,6 +41,42 @@ def test_fused_run_is_independent_of_numbering(name):
     assert g0.scalars() == g1.scalars()
 
 
+def _shuffled(m, seed):
+    """The same model with its elements renumbered at random."""
+    import copy
+    m2 = copy.deepcopy(m)
+    perm = np.random.default_rng(seed).permutation(m.nelem)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(m.nelem)
+    m2.conn = np.ascontiguousarray(m.conn[perm])
+    for k in ("stress0", "strain0", "stressyy0", "plstrain0", "elem_visc", "elem_cohesion", "elemmarkers"):
+        v = getattr(m, k)
+        if v is not None:
+            setattr(m2, k, np.ascontiguousarray(v[perm]))
+    b, e, f = m.bfacets
+    m2.bfacets = (b, inv[e].astype(e.dtype), f)
+    return m2
+
+
+@pytest.mark.parametrize("name", ["evp3", "evp2"])
+def test_direct_gathers_equal_staged_gathers(name):
+    """Passes A and E stage a block's node records in LDS when the block's
+    256 elements touch at most 256 distinct nodes, and gather directly
+    otherwise.  Randomly numbered elements in the host's order overflow every
+    block (direct gathers); the curve order of the same mesh stages them."""
+    m = _shuffled(SC.build(name), 3)
+    blocks = [len(np.unique(m.conn[b:b + 256])) for b in range(0, m.nelem - 255, 256)]
+    print(name, m.nelem, "elements; distinct nodes per block in host order >=", min(blocks), flush=True)
+    assert min(blocks) > 256
+    g0, g1 = _make(m, 0), _make(m, 1)
+    g0.run(25)
+    g1.run(25)
+    _same(g0.download(), g1.download())
+    g0.run(10, abi.DES_MIXED)
+    g1.run(10, abi.DES_MIXED)
+    _same(g0.download(), g1.download())
+
+
 @pytest.mark.parametrize("name", ["evp3", "mat2"])
 def test_per_function_steps_are_independent_of_numbering(name):
     m = SC.build(name)
