@@ -1298,8 +1298,9 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     if (DEFER && ND == 3 && (s.P->rheol_type == DES_RH_EP || s.P->rheol_type == DES_RH_EVP)) {
         defer = plastic_deferred3(s, e, dt, Tmean, vol, s.volume_old[e], st, edot, pl, dpl, dp, trial, old_s);
     } else {
-        ok = rheology_after_strain<ND>(s, e, dt, total_dv, Tmean, vol, s.volume_old[e], st, syy, edot, pl, dpl,
-                                       dp);
+        // 3D plastic rheologies are deferred above: their inline return is not compiled in here
+        ok = rheology_after_strain<ND, !(DEFER && ND == 3)>(s, e, dt, total_dv, Tmean, vol, s.volume_old[e], st,
+                                                           syy, edot, pl, dpl, dp);
     }
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     store_tensor<ND>(s.stress, E, e, st);

@@ -1002,8 +1002,11 @@ template <int ND> __device__ __forceinline__ void plastic_return(double bulkm, d
 // The rheology part of update_stress (rheology.cxx:542-641) for one element
 // whose strain rate is already anti-locked and whose strain is already
 // updated; total_dv = trace(strain) for the viscous rheology.  Returns false
-// for an unknown rheology.
-template <int ND> __device__ __forceinline__ bool rheology_after_strain(const DevState &st, int e, double dt,
+// for an unknown rheology.  PLASTIC = false leaves the elasto-plastic and
+// elasto-visco-plastic branches out (returning false), for a caller that
+// handles those rheologies itself: the Mohr-Coulomb return and its eigen
+// solve then do not set the kernel's register budget.
+template <int ND, bool PLASTIC = true> __device__ __forceinline__ bool rheology_after_strain(const DevState &st, int e, double dt,
                                                                        double total_dv, double Tmean,
                                                                        double vol, double vol_old,
                                                                        double *s, double &syy,
@@ -1037,6 +1040,10 @@ template <int ND> __device__ __forceinline__ bool rheology_after_strain(const De
         break;
     }
     case DES_RH_EP: {
+        if constexpr (!PLASTIC) {
+            ok = false;
+            break;
+        }
         double depls = 0;
         double bulkm = harmonic_mean(st, P->bulk_modulus, e);
         double shearm = harmonic_mean(st, P->shear_modulus, e);
@@ -1049,6 +1056,10 @@ template <int ND> __device__ __forceinline__ bool rheology_after_strain(const De
         break;
     }
     case DES_RH_EVP: {
+        if constexpr (!PLASTIC) {
+            ok = false;
+            break;
+        }
         double depls = 0;
         double bulkm = harmonic_mean(st, P->bulk_modulus, e);
         double shearm = harmonic_mean(st, P->shear_modulus, e);
