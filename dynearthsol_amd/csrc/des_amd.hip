@@ -1750,6 +1750,11 @@ struct des_ctx {
     // step's deferred-rotation flag
     std::vector<double *> adj_tape;
     std::vector<int> adj_tape_rot;
+    // checkpointed run: one window's records (8 buffers per step), and
+    // whether they still hold the run's last window, recorded by the forward
+    std::vector<double *> adj_win;
+    std::vector<int> adj_win_rot;
+    bool adj_win_last = false;
 };
 
 // host description of a batched group's concatenated mesh

@@ -259,15 +259,18 @@ int des_set_tuning(des_ctx *ctx, int variant);
    Other configurations return 1 (DES_ERR_INPUT). */
 
 /* nsteps fused FP64 steps from the current state, keeping a device checkpoint
-   every `window` steps; leaves the context at the final state.  window = 0
-   tapes the run instead: every step's records stay on the device
-   ((14 nnode + 8 nelem) doubles per step, refused with DES_ERR_INPUT beyond 80%
-   of free device memory), and the backward recomputes nothing. */
+   every `window` steps and the records of one window ((14 nnode + 8 nelem)
+   doubles per step, window + 1 steps; the last window's are recorded here);
+   leaves the context at the final state.  window = 0 tapes the run instead:
+   every step's records stay on the device (nsteps + 1 steps), and the
+   backward recomputes nothing.  Records beyond 80% of free device memory are
+   refused with DES_ERR_INPUT. */
 int des_adj_forward(des_ctx *ctx, int nsteps, int window);
 
 /* coord_bar (nnode x ndims, AoS) = d loss / d final coordinates.  Recomputes
-   each window from its checkpoint (taped run: reads the tape) and runs the
-   reverse sweep; writes
+   each window from its checkpoint (the first call takes the last window's
+   records from the forward; taped run: reads the tape) and runs the reverse
+   sweep; may be called again for another coord_bar; writes
    visc_bar[e] = d loss / d elem_visc[e] (0 where the viscosity is clamped to
    the min / max) and cohesion_bar[e] = d loss / d elem_cohesion[e] (either
    may be null).  The context is left at the final state of the run. */

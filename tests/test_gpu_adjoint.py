@@ -126,6 +126,16 @@ def test_window_size_does_not_change_the_gradient():
     assert np.array_equal(v1, v3) and np.array_equal(h1, h3)
 
 
+def test_repeated_backward_gives_the_same_gradient():
+    """The forward records its last window, which the first backward uses
+    without recomputing it and then overwrites: a second backward recomputes
+    every window and must give the same gradient bitwise."""
+    m, cb = _case("plastic", perturb=30e6, coh_scale=0.2)
+    g, v1, h1 = _gpu_grad(m, cb, 11, 4)
+    v2, h2 = g.adj_backward(cb)
+    assert np.array_equal(v1, v2) and np.array_equal(h1, h2)
+
+
 def test_taped_run_matches_checkpointed_run():
     """window=0 keeps every step's records from the forward: the same
     gradient bitwise, the same final state, and a second backward over the
