@@ -1730,6 +1730,8 @@ struct des_ctx {
     int avg_interval = 0;          // Output::average_interval (0: averaged fields off)
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
     std::vector<int> eh, nh;       // device element / node -> host index (model-local in a group)
+    double *xfer = nullptr;        // device staging of host-order transfers, grown on demand
+    size_t xfer_cap = 0;           // its size in doubles
     std::vector<int> h_conn;       // host copies for output frames
     std::vector<unsigned> h_bcflag;
     int passc_variant = -1;        // waves-per-EU variant of pass C chosen by the autotuner
@@ -1824,19 +1826,66 @@ bool facet_loaded(const des_params &P, int i)
 
 // host AoS [n][k] <-> device SoA [k][stride], rows off .. off+n-1; device
 // row off + i holds host row perm[i] (i without perm)
+// Host-order AoS (row perm[i], k values per row) <-> device SoA (component j
+// of device row i at soa[j * stride + i]), one thread per device row
+__global__ __launch_bounds__(BLOCK) void k_soa_to_aos(const double *soa, long stride, long n, int k,
+                                                      const int *perm, double *aos)
+{
+    const long i = blockIdx.x * (long)BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const long h = perm ? perm[i] : i;
+    for (int j = 0; j < k; ++j) aos[h * k + j] = soa[j * stride + i];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_aos_to_soa(const double *aos, long n, int k, const int *perm,
+                                                      double *soa, long stride)
+{
+    const long i = blockIdx.x * (long)BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const long h = perm ? perm[i] : i;
+    for (int j = 0; j < k; ++j) soa[j * stride + i] = aos[h * k + j];
+}
+
+// the device copy of a host permutation (a range of eh or nh)
+const int *device_perm(const des_ctx *ctx, const int *perm)
+{
+    const auto in = [&](const std::vector<int> &v) {
+        return !v.empty() && perm >= v.data() && perm < v.data() + v.size();
+    };
+    if (in(ctx->eh)) return ctx->s.ehost + (perm - ctx->eh.data());
+    if (in(ctx->nh)) return ctx->s.nhost + (perm - ctx->nh.data());
+    return nullptr;
+}
+
+int ensure_xfer(des_ctx *ctx, size_t n)
+{
+    if (ctx->xfer_cap >= n) return DES_OK;
+    if (ctx->xfer) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipFree(ctx->xfer));
+        ctx->xfer = nullptr;
+        ctx->xfer_cap = 0;
+    }
+    HIPCHK(hipMalloc(&ctx->xfer, std::max<size_t>(n, 1) * sizeof(double)));
+    ctx->xfer_cap = n;
+    return DES_OK;
+}
+
+// Transfers in the host's numbering: the rows are permuted and interleaved
+// on the device, and one copy crosses PCIe.
 int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0,
                const int *perm = nullptr)
 {
-    if (!src) return DES_OK;
+    if (!src || n <= 0) return DES_OK;
     if (stride < 0) stride = n;
-    std::vector<double> tmp((size_t)n * k);
-    for (long i = 0; i < n; ++i) {
-        const size_t h = perm ? (size_t)perm[i] : (size_t)i;
-        for (int j = 0; j < k; ++j) tmp[(size_t)j * n + i] = src[h * k + j];
-    }
-    for (int j = 0; j < k; ++j)
-        HIPCHK(hipMemcpyAsync(dst + j * stride + off, tmp.data() + (size_t)j * n, n * sizeof(double),
-                              hipMemcpyHostToDevice, ctx->stream));
+    const int *dp = device_perm(ctx, perm);
+    if (perm && !dp) return fail(ctx, DES_ERR_INPUT, "upload: unknown row order");
+    int rc;
+    if ((rc = ensure_xfer(ctx, (size_t)n * k))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->xfer, src, sizeof(double) * n * k, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblocks(n)), dim3(BLOCK), 0, ctx->stream, ctx->xfer, n, k, dp,
+                       dst + off, stride);
+    if ((rc = check_launch(ctx))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
 }
@@ -1844,17 +1893,17 @@ int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long
 int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0,
                  const int *perm = nullptr)
 {
-    if (!dst) return DES_OK;
+    if (!dst || n <= 0) return DES_OK;
     if (stride < 0) stride = n;
-    std::vector<double> tmp((size_t)n * k);
-    for (int j = 0; j < k; ++j)
-        HIPCHK(hipMemcpyAsync(tmp.data() + (size_t)j * n, src + j * stride + off, n * sizeof(double),
-                              hipMemcpyDeviceToHost, ctx->stream));
+    const int *dp = device_perm(ctx, perm);
+    if (perm && !dp) return fail(ctx, DES_ERR_INPUT, "download: unknown row order");
+    int rc;
+    if ((rc = ensure_xfer(ctx, (size_t)n * k))) return rc;
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nblocks(n)), dim3(BLOCK), 0, ctx->stream, src + off, stride, n, k, dp,
+                       ctx->xfer);
+    if ((rc = check_launch(ctx))) return rc;
+    HIPCHK(hipMemcpyAsync(dst, ctx->xfer, sizeof(double) * n * k, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (long i = 0; i < n; ++i) {
-        const size_t h = perm ? (size_t)perm[i] : (size_t)i;
-        for (int j = 0; j < k; ++j) dst[h * k + j] = tmp[(size_t)j * n + i];
-    }
     return DES_OK;
 }
 
@@ -2411,6 +2460,7 @@ void des_destroy(des_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void *p : ctx->allocs) (void)hipFree(p);
     for (void *p : ctx->adj_allocs) (void)hipFree(p);
+    if (ctx->xfer) (void)hipFree(ctx->xfer);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
