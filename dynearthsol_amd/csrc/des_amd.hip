@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <initializer_list>
+#include <thread>
 #include <vector>
 
 #include "des_amd.h"
@@ -2040,8 +2041,28 @@ uint64_t hilbert_index(uint32_t X[3], int nd, int bits)
     return k;
 }
 
+// f(t) for t = 0 .. T-1 on T host threads
+template <class F> void run_threads(int T, F f)
+{
+    if (T <= 1) {
+        f(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(f, t);
+    for (auto &x : th) x.join();
+}
+
+// host threads for n items (at most 16, at least 64 k items each)
+int host_threads(long n)
+{
+    const long hw = std::max(1u, std::thread::hardware_concurrency());
+    return (int)std::max(1L, std::min({hw, 16L, n / 65536}));
+}
+
 // perm[device index] = host index, by Hilbert index of the n points x[n][nd]
-// (one scale for every axis, so the curve's cells are cubes)
+// (one scale for every axis, so the curve's cells are cubes).  Keys and the
+// sort run on host threads: chunks sorted in parallel, then merged pairwise.
 void curve_order(const double *x, long n, int nd, std::vector<int> &perm)
 {
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
@@ -2055,16 +2076,29 @@ void curve_order(const double *x, long n, int nd, std::vector<int> &perm)
     const double cells = (double)((1u << (nd == 3 ? 21 : 31)) - 1);
     const double scale = ext > 0 ? cells / ext : 0;
     std::vector<std::pair<uint64_t, int>> key(n);
-    for (long i = 0; i < n; ++i) {
-        uint32_t X[3] = {0, 0, 0};
-        for (int d = 0; d < nd; ++d) {
-            double q = std::floor((x[i * nd + d] - lo[d]) * scale);
-            q = std::min(std::max(q, 0.0), cells);
-            X[d] = (uint32_t)q;
+    const int T = host_threads(n);
+    std::vector<long> b(T + 1);
+    for (int t = 0; t <= T; ++t) b[t] = n * t / T;
+    run_threads(T, [&](int t) {
+        for (long i = b[t]; i < b[t + 1]; ++i) {
+            uint32_t X[3] = {0, 0, 0};
+            for (int d = 0; d < nd; ++d) {
+                double q = std::floor((x[i * nd + d] - lo[d]) * scale);
+                q = std::min(std::max(q, 0.0), cells);
+                X[d] = (uint32_t)q;
+            }
+            key[i] = {hilbert_index(X, nd, nd == 3 ? 21 : 31), (int)i};
         }
-        key[i] = {hilbert_index(X, nd, nd == 3 ? 21 : 31), (int)i};
+        std::sort(key.begin() + b[t], key.begin() + b[t + 1]);
+    });
+    for (int w = 1; w < T; w *= 2) {
+        const int pairs = (T + 2 * w - 1) / (2 * w);
+        run_threads(pairs, [&](int p) {
+            const int lo_ = 2 * w * p, mid = lo_ + w, hi_ = std::min(lo_ + 2 * w, T);
+            if (mid < hi_)
+                std::inplace_merge(key.begin() + b[lo_], key.begin() + b[mid], key.begin() + b[hi_]);
+        });
     }
-    std::sort(key.begin(), key.end());
     perm.resize(n);
     for (long i = 0; i < n; ++i) perm[i] = key[i].second;
 }
@@ -2180,14 +2214,25 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
     for (int n = 0; n < N; ++n) off[n + 1] += off[n];
     std::vector<int> ent(std::max(off[N], 1));
     {
-        std::vector<int> fill(off.begin(), off.end() - 1);
+        // elements visited in ascending host index, so that every node lists
+        // its elements in ascending host order, the reference's summation
+        // order (the elements of a node all belong to one model of a group)
+        int hmax = 0;
         for (int e = 0; e < E; ++e)
-            if (live[e])
-                for (int i = 0; i < npe; ++i) ent[fill[m->connectivity[(size_t)e * npe + i]]++] = e * npe + i;
-        // a node's elements in ascending host order: the reference's summation order
-        for (int n = 0; n < N; ++n)
-            std::sort(ent.begin() + off[n], ent.begin() + off[n + 1],
-                      [&](int a, int b) { return eh[a / npe] < eh[b / npe]; });
+            if (live[e]) hmax = std::max(hmax, eh[e] + 1);
+        std::vector<int> hoff(hmax + 1, 0);
+        for (int e = 0; e < E; ++e)
+            if (live[e]) hoff[eh[e] + 1]++;
+        for (int h = 0; h < hmax; ++h) hoff[h + 1] += hoff[h];
+        const int nlive = hoff[hmax];
+        std::vector<int> byh(std::max(nlive, 1));
+        for (int e = 0; e < E; ++e)
+            if (live[e]) byh[hoff[eh[e]]++] = e;
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int k = 0; k < nlive; ++k) {
+            const int e = byh[k];
+            for (int i = 0; i < npe; ++i) ent[fill[m->connectivity[(size_t)e * npe + i]]++] = e * npe + i;
+        }
     }
     ctx->eh = eh;
     ctx->nh = nh;
@@ -2212,7 +2257,7 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
         // nodes as positions in that list (blocks above STAGE_MAX: count -1)
         const int neb = nblocks(E);
         std::vector<int2> stage(neb);
-        std::vector<int> snode, u;
+        std::vector<int> snode, u, mark(N, -1), pos(N, 0);
         std::vector<unsigned> lidx(E, 0);
         snode.reserve((size_t)E);
         for (int b = 0; b < neb; ++b) {
@@ -2220,21 +2265,24 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
             u.clear();
             for (int e = e0; e < e1; ++e)
                 if (live[e])
-                    for (int i = 0; i < npe; ++i) u.push_back(m->connectivity[(size_t)e * npe + i]);
-            std::sort(u.begin(), u.end());
-            u.erase(std::unique(u.begin(), u.end()), u.end());
+                    for (int i = 0; i < npe; ++i) {
+                        const int nd_ = m->connectivity[(size_t)e * npe + i];
+                        if (mark[nd_] != b) {
+                            mark[nd_] = b;
+                            u.push_back(nd_);
+                        }
+                    }
             if ((int)u.size() > STAGE_MAX) {
                 stage[b] = make_int2((int)snode.size(), -1);
                 continue;
             }
+            std::sort(u.begin(), u.end());
+            for (int k = 0; k < (int)u.size(); ++k) pos[u[k]] = k;
             stage[b] = make_int2((int)snode.size(), (int)u.size());
             for (int e = e0; e < e1; ++e)
                 if (live[e])
-                    for (int i = 0; i < npe; ++i) {
-                        const int k = (int)(std::lower_bound(u.begin(), u.end(),
-                                                             m->connectivity[(size_t)e * npe + i]) - u.begin());
-                        lidx[e] |= (unsigned)k << (8 * i);
-                    }
+                    for (int i = 0; i < npe; ++i)
+                        lidx[e] |= (unsigned)pos[m->connectivity[(size_t)e * npe + i]] << (8 * i);
             snode.insert(snode.end(), u.begin(), u.end());
         }
         if (snode.empty()) snode.push_back(0);
