@@ -312,6 +312,24 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_edvoldt(DevState s)
     s.edvoldt[e] = dj / Dim<ND>::NPE;
 }
 
+// Non-temporal variants for arrays a pass streams once per step and no pass
+// reads again before the next step's same pass (strain, plastic strain, the
+// SoA velocity and coordinates): they keep the L2 for the node records and
+// contributions that the following passes gather.
+template <int ND> __device__ __forceinline__ void load_tensor_nt(const double *a, long E, int e,
+                                                               double t[Dim<ND>::NSTR])
+{
+#pragma unroll
+    for (int k = 0; k < Dim<ND>::NSTR; ++k) t[k] = __builtin_nontemporal_load(a + k * E + e);
+}
+
+template <int ND> __device__ __forceinline__ void store_tensor_nt(double *a, long E, int e,
+                                                                const double t[Dim<ND>::NSTR])
+{
+#pragma unroll
+    for (int k = 0; k < Dim<ND>::NSTR; ++k) __builtin_nontemporal_store(t[k], a + k * E + e);
+}
+
 template <int ND> __device__ __forceinline__ void load_tensor(const double *a, long E, int e,
                                                             double t[Dim<ND>::NSTR])
 {
@@ -1274,11 +1292,11 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
 #pragma unroll
         for (int i = 0; i < ND; ++i) edot[i] += (edv - div) / ND;
         double es[NSTR];
-        load_tensor<ND>(s.strain, E, e, es);
+        load_tensor_nt<ND>(s.strain, E, e, es);
         if (rot) jaumann<ND>(es, dtp, w);
 #pragma unroll
         for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dt;
-        store_tensor<ND>(s.strain, E, e, es);
+        store_tensor_nt<ND>(s.strain, E, e, es);
         total_dv = trace<ND>(es);
     }
     if (fl.store_aux) {
@@ -1289,7 +1307,8 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     load_tensor<ND>(s.stress, E, e, st);
     if (fl.rec_stress) store_tensor<ND>(fl.rec_stress, E, e, st);
     if (rot) jaumann<ND>(st, dtp, w);
-    double syy = s.stressyy[e], pl = s.plstrain[e], dpl = s.delta_plstrain[e], dp;
+    double syy = s.stressyy[e], dp;
+    double pl = __builtin_nontemporal_load(s.plstrain + e), dpl = __builtin_nontemporal_load(s.delta_plstrain + e);
     if (fl.rec_plstrain) fl.rec_plstrain[e] = pl;
     if (fl.rec_volume_old) fl.rec_volume_old[e] = s.volume_old[e];
     const double vol = s.volume[e];
@@ -1306,8 +1325,8 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     store_tensor<ND>(s.stress, E, e, st);
     if (s.P->is_plane_strain) s.stressyy[e] = syy;
-    s.plstrain[e] = pl;
-    s.delta_plstrain[e] = dpl;
+    __builtin_nontemporal_store(pl, s.plstrain + e);
+    __builtin_nontemporal_store(dpl, s.delta_plstrain + e);
     s.dpressure[e] = dp;
     s.escal[e] = dp * vol;          // NMD_stress's dp * volume (geometry.cxx:206)
     if constexpr (DEFER) {
@@ -1437,7 +1456,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
         step_end_scalars(s.sc, s.P->rheol_type);
     double f[ND], v[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) v[d] = s.vel[d * s.N + n];
+    for (int d = 0; d < ND; ++d) v[d] = __builtin_nontemporal_load(s.vel + d * s.N + n);
     node_force<ND>(s, n, f, v);
     if (fl.store_aux) {
 #pragma unroll
@@ -1456,9 +1475,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
     double *rn = s.nrec + (long)NREC * n;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        s.vel[d * s.N + n] = v[d];
-        const double xn = s.coord[d * s.N + n] + v[d] * dt;
-        s.coord[d * s.N + n] = xn;
+        __builtin_nontemporal_store(v[d], s.vel + d * s.N + n);
+        const double xn = __builtin_nontemporal_load(s.coord + d * s.N + n) + v[d] * dt;
+        __builtin_nontemporal_store(xn, s.coord + d * s.N + n);
         rn[NrecSlot<ND>::X + d] = xn;
         rn[NrecSlot<ND>::V + d] = v[d];
     }
