@@ -1237,17 +1237,18 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
     } else {
         s.volume_n[n] = vn;
         s.mass[n] = m;
-        s.tmass[n] = tm;
+        __builtin_nontemporal_store(tm, s.tmass + n);      // read again only by a run's first step
     }
     double *rn = s.nrec + (long)NREC * n;
     double Tnew = 0;
-    if (thermal) {
-        Tnew = temperature_new(s, n, s.T[n], tdot, s.sc->dt, tm);
-        s.T[n] = Tnew;
+    if (thermal) {      // the SoA temperature: read only by this pass (the others read the records)
+        Tnew = temperature_new(s, n, __builtin_nontemporal_load(s.T + n), tdot, s.sc->dt, tm);
+        __builtin_nontemporal_store(Tnew, s.T + n);
         rn[NrecSlot<ND>::T] = Tnew;
     }
+    // dvoldt goes to the records only: ntmp is left to pass D (NMD_stress's
+    // dp_nd, its last writer in the reference's step)
     const double dvn = dj / vn;
-    s.ntmp[n] = dvn;
     rn[NrecSlot<ND>::AUX] = dvn;
     if (fl.rec_nrec) {
         double *r = fl.rec_nrec + (long)NREC * n;
@@ -1417,7 +1418,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s, 
     }
     for (; p < p1; ++p) sum += s.escal[s.csr_ent[p] / NPE];
     const double dpn = sum / s.volume_n[n];
-    s.ntmp[n] = dpn;
+    __builtin_nontemporal_store(dpn, s.ntmp + n);          // read only at a sync
     s.nrec[(long)NREC * n + NrecSlot<ND>::AUX] = dpn;
     if (fl.rec_dpn) fl.rec_dpn[n] = dpn;
 }
