@@ -1542,9 +1542,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     double p = s.pres[e];
     float sd[NSTR], es[NSTR], edot[NSTR];
 #pragma unroll
-    for (int k = 0; k < NSTR; ++k) {
+    for (int k = 0; k < NSTR; ++k) {       // strain and plastic strain: non-temporal streams
         sd[k] = s.sdev[k * E + e];
-        es[k] = s.strain32[k * E + e];
+        es[k] = __builtin_nontemporal_load(s.strain32 + k * E + e);
     }
     float rs[NSTR], re[NSTR];
 #pragma unroll
@@ -1567,7 +1567,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
 #pragma unroll
     for (int i = 0; i < NPE; ++i) edv += aux[i];
     edv = edv / NPE;
-    float pl = s.pl32[e], dpl = s.dpl32[e];
+    float pl = __builtin_nontemporal_load(s.pl32 + e), dpl = __builtin_nontemporal_load(s.dpl32 + e);
     double dp;
     const float Tm = (float)node_mean<ND>(Tn);
     bool deferred;
@@ -1576,13 +1576,13 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
                                                sd, rs, es, re, edot, pl, dpl, dp, deferred, rec);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
 #pragma unroll
-    for (int k = 0; k < NSTR; ++k) s.strain32[k * E + e] = es[k];
+    for (int k = 0; k < NSTR; ++k) __builtin_nontemporal_store(es[k], s.strain32 + k * E + e);
     if (!deferred) {
         s.pres[e] = p;
 #pragma unroll
         for (int k = 0; k < NSTR; ++k) s.sdev[k * E + e] = sd[k];
-        s.pl32[e] = pl;
-        s.dpl32[e] = dpl;
+        __builtin_nontemporal_store(pl, s.pl32 + e);
+        __builtin_nontemporal_store(dpl, s.dpl32 + e);
         s.dpressure[e] = dp;
         s.escal[e] = dp * vol;
     }
