@@ -43,8 +43,8 @@ PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
 # bytes moved): (2 FETCH + WRITE) * 1024.  Measured on the 3d-fp64 workload,
 # 1019592 tetrahedra; the commands are in profiles/.
 TRAFFIC = {
-    "3d-fp64": {"k_pass_a": 92692281, "k_pass_b": 231946312, "k_pass_c": 300087143, "k_pass_d": 35101436,
-                "k_pass_e": 239227893, "k_pass_f": 154904994},
+    "3d-fp64": {"k_pass_a": 92690817, "k_pass_b": 225450346, "k_pass_c": 302206372, "k_pass_d": 35101817,
+                "k_pass_e": 239278641, "k_pass_f": 154240694},
 }
 
 WORKLOADS = {
