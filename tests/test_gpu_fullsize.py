@@ -1,8 +1,10 @@
-"""The benchmark's own model at full size (3d-fp64: 100x100x10 km, 1.02 M
-tetrahedra, elasto-visco-plastic with a weak slab and plastic-strain noise):
-the fused run against the oracle, and properties that hold at any size --
-fused equals per-function bitwise, the device numbering is invisible, mass
-and nodal volume are sums of the element terms."""
+"""The workloads at full size.  The benchmark's model (3d-fp64: 100x100x10
+km, 1.02 M tetrahedra, elasto-visco-plastic with a weak slab and
+plastic-strain noise): the fused run against the oracle, and properties that
+hold at any size -- fused equals per-function bitwise, the device numbering
+is invisible, mass and nodal volume are sums of the element terms.  The
+adjoint model (200 k tetrahedra) and ensemble-sized members: taped =
+checkpointed, linearity of the gradient, batched = alone."""
 import numpy as np
 import pytest
 
@@ -82,3 +84,50 @@ def test_benchmark_model_mass_and_nodal_volume_sums(bench_model):
     vol = f["volume"].sum()
     assert abs(f["volume_n"].sum() / (4 * vol) - 1) < 1e-12       # volume_n: the whole volume of each element
     assert abs(f["mass"].sum() / (rho_eff * vol) - 1) < 1e-12
+
+
+def test_adjoint_model_taped_equals_checkpointed_and_is_linear():
+    """The adjoint workload's model (200 k tetrahedra, plastic-enabled): the
+    taped gradient equals the checkpointed one bitwise, and the reverse sweep
+    is linear in the loss weights on the top surface."""
+    from dynearthsol_amd.lib import DesModel
+    m = M.config_adjoint("plastic")
+    print("adjoint model:", m.nelem, "elements", flush=True)
+    top = (m.bcflag & 32) != 0
+    rng = np.random.default_rng(1)
+    c1, c2 = np.zeros((m.nnode, 3)), np.zeros((m.nnode, 3))
+    c1[top, 2] = rng.standard_normal(int(top.sum()))
+    c2[top, 2] = rng.standard_normal(int(top.sum()))
+    gw = DesModel(m)
+    gw.adj_forward(30, 8)
+    v1, h1 = gw.adj_backward(c1)
+    v2, h2 = gw.adj_backward(c2)
+    v12, h12 = gw.adj_backward(2 * c1 + 3 * c2)
+    gt = DesModel(m)
+    gt.adj_forward(30, 0)
+    vt, ht = gt.adj_backward(c1)
+    assert np.array_equal(v1, vt) and np.array_equal(h1, ht)
+    for a, b in ((v12, 2 * v1 + 3 * v2), (h12, 2 * h1 + 3 * h2)):
+        err = SC.rel_err(a, b)
+        print("linearity:", err, flush=True)
+        assert err < 1e-10
+
+
+def test_ensemble_members_batched_equal_alone():
+    """Six members drawn like the ensemble workload's (3 2D with 1k-50k
+    triangles, 3 3D with 10k-200k tetrahedra), mixed precision: each ends
+    bitwise equal to the same model run on its own context."""
+    from dynearthsol_amd import abi
+    from dynearthsol_amd.lib import DesBatch, DesModel
+    models = M.config_ensemble(n2d=3, n3d=3)
+    print("members:", [mm.nelem for mm in models], flush=True)
+    bt = DesBatch(models)
+    bt.run(20, abi.DES_MIXED)
+    bt.sync()
+    for k, mm in enumerate(models):
+        g = DesModel(mm)
+        g.run(20, abi.DES_MIXED)
+        a, b = bt.download(k), g.download()
+        for name in a:
+            assert np.array_equal(a[name], b[name]), (k, name)
+        assert bt.scalars(k) == g.scalars(), k
