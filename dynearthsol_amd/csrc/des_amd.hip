@@ -2487,6 +2487,7 @@ int validate_mesh(const des_params *p, const des_mesh *m)
     if (p->nmat < 1 || p->nmat > DES_MAX_MAT) return DES_ERR_INPUT;
     if (m->nnode <= 0 || m->nelem <= 0 || !m->connectivity || !m->coord || !m->bcflag) return DES_ERR_MESH;
     const int nd = p->ndims, npe = nd + 1;
+    if ((long)m->nelem * npe >= (1L << 31)) return DES_ERR_MESH;      // CSR entries e * npe + i are 32-bit
     for (long k = 0; k < (long)m->nelem * npe; ++k)
         if (m->connectivity[k] < 0 || m->connectivity[k] >= m->nnode) return DES_ERR_MESH;
     if (m->nbfacet > 0 && (!m->bfacet_bdry || !m->bfacet_elem || !m->bfacet_face)) return DES_ERR_MESH;

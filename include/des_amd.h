@@ -29,7 +29,7 @@ enum des_status {
     DES_OK = 0,
     DES_ERR_INPUT = 1,     /* invalid parameter, unknown rheology or damping option */
     DES_ERR_IO = 2,
-    DES_ERR_MESH = 10,     /* inconsistent mesh (bad connectivity, node out of range) */
+    DES_ERR_MESH = 10,     /* inconsistent mesh (bad connectivity, node out of range), or nelem * (ndims + 1) >= 2^31 */
     DES_ERR_RUNTIME = 11,  /* dt <= 0, non-finite coordinates */
     DES_ERR_ASSERT = 12,   /* missing edge vector of a boundary intersection */
     DES_ERR_HIP = 100,     /* HIP runtime failure (message in des_last_error) */
