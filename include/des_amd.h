@@ -146,7 +146,8 @@ typedef struct des_ctx des_ctx;
 /* Create a model on the current HIP device: uploads the mesh, builds the
    node->element CSR (create_support, mesh.cxx:1707-1721) and the per-node
    boundary-facet lists.  Fields start at zero, as allocate_variables leaves
-   them (fields.cxx:10-48). */
+   them (fields.cxx:10-48).  On failure *out may hold a partly built context
+   whose des_last_error explains it: release it with des_destroy. */
 int des_create(const des_params *p, const des_mesh *m, des_ctx **out);
 void des_destroy(des_ctx *ctx);
 /* Device numbering of the meshes given to later des_create / des_batch_create
@@ -290,6 +291,8 @@ void des_adj_release(des_ctx *ctx);
    model nsteps times, with compute_dt every 10 steps per model. */
 typedef struct des_batch des_batch;
 
+/* On failure *out may hold a partly built batch (des_batch_last_error):
+   release it with des_batch_destroy. */
 int des_batch_create(int nmodels, const des_params *params, const des_mesh *meshes, des_batch **out);
 void des_batch_destroy(des_batch *b);
 const char *des_batch_last_error(const des_batch *b);
