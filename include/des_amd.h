@@ -239,7 +239,9 @@ int des_run(des_ctx *ctx, int nsteps, int precision);
 int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[10], int counts[10]);
 
 /* Wait for all work queued on the context's stream; returns the first device
-   error the step detected (dt <= 0, missing edge vector). */
+   error the step detected (dt <= 0, missing edge vector, unknown rheology).
+   des_run only queues work: its device-side failures surface here (or at the
+   next compute_dt, every 10 steps). */
 int des_sync(des_ctx *ctx);
 
 /* The pass-C variant (minimum waves per SIMD 1..4 as 0..3) that the autotuner
