@@ -2837,6 +2837,7 @@ int des_get_scalars(des_ctx *ctx, double *dt, double *time, int *steps)
     if (dt) *dt = h.dt;
     if (time) *time = h.time;
     if (steps) *steps = h.steps;
+    if (h.err) return fail(ctx, h.err, "device reported an error");
     return DES_OK;
 }
 

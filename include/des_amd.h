@@ -190,6 +190,8 @@ int des_upload_elemmarkers(des_ctx *ctx, const int *counts);
 /* Scalar state: var.dt, var.time, var.steps, var.compensation_pressure. */
 int des_set_scalars(des_ctx *ctx, double dt, double time, int steps,
                     double compensation_pressure);
+/* des_get_scalars waits for the queued work and, like des_sync, returns the
+   first device error the steps detected (the scalars are filled either way). */
 int des_get_scalars(des_ctx *ctx, double *dt, double *time, int *steps);
 
 /* ---- per-function entry points (same semantics as the named function) -- */
