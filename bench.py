@@ -40,11 +40,14 @@ PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
 # dispatch, averaged over 600 dispatches), each scaled by what the counter
 # reads for a plain 1 GiB copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the
-# bytes moved): (2 FETCH + WRITE) * 1024.  Measured on the 3d-fp64 workload,
-# 1019592 tetrahedra; the commands are in profiles/.
+# bytes moved): (2 FETCH + WRITE) * 1024.  Measured on the 3d-fp64 workload
+# (1019592 tetrahedra) and the 3d-mixed workload (8088336 tetrahedra, mixed
+# precision); the commands are in profiles/.
 TRAFFIC = {
     "3d-fp64": {"k_pass_a": 92690817, "k_pass_b": 225450346, "k_pass_c": 302206372, "k_pass_d": 35101817,
                 "k_pass_e": 239278641, "k_pass_f": 154240694},
+    "3d-mixed": {"k_pass_a": 736108774, "k_pass_b": 1952328345, "k_pass_c_mixed": 1627773140,
+                 "k_pass_d": 639150265, "k_pass_e_mixed": 1751009560, "k_pass_f": 1287281115},
 }
 
 WORKLOADS = {
