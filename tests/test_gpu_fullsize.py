@@ -131,3 +131,23 @@ def test_ensemble_members_batched_equal_alone():
         for name in a:
             assert np.array_equal(a[name], b[name]), (k, name)
         assert bt.scalars(k) == g.scalars(), k
+
+
+def test_cpu_configuration_long_trajectory_matches_oracle():
+    """The reference's CPU-runnable configuration (2D extension, 100x10 km,
+    about 4 k triangles, elasto-visco-plastic): 300 fused steps in chunks,
+    across thirty dt updates, against the oracle's 300 steps, every field
+    within 1e-10 (2D runs do not go through the 3x3 eigen solver)."""
+    m = M.config_benchmark("2d-extension")
+    print("2d-extension:", m.nelem, "triangles", flush=True)
+    g = _gpu(m)
+    for n in (100, 150, 50):
+        g.run(n)
+    o = OracleState(m)
+    o.step(300)
+    got = g.download()
+    err = {k: SC.rel_err(got[k], o.f[k]) for k in got if k != "stressyy"}
+    worst = max(err, key=err.get)
+    print("worst field", worst, f"{err[worst]:.2e}", flush=True)
+    assert err[worst] < 1e-10, err
+    assert g.scalars()[2] == 300
