@@ -3538,6 +3538,7 @@ int des_batch_get_scalars(des_batch *b, int model, double *dt, double *time, int
     if (dt) *dt = h.dt;
     if (time) *time = h.time;
     if (steps) *steps = h.steps;
+    if (h.err) return bfail(b, h.err, "device reported an error in model " + std::to_string(model));
     return DES_OK;
 }
 

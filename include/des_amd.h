@@ -304,6 +304,7 @@ int des_batch_upload_fields(des_batch *b, int model, const des_fields *f);
 int des_batch_download_fields(des_batch *b, int model, des_fields *f);
 int des_batch_set_scalars(des_batch *b, int model, double dt, double time, int steps,
                           double compensation_pressure);
+/* As des_get_scalars: also returns the model's first pending device error. */
 int des_batch_get_scalars(des_batch *b, int model, double *dt, double *time, int *steps);
 int des_batch_init_derived(des_batch *b);            /* des_init_derived for every model */
 int des_batch_apply_vbcs(des_batch *b);              /* des_apply_vbcs for every model */

@@ -52,3 +52,15 @@ def test_context_still_usable_after_a_refused_adjoint():
     assert np.array_equal(g.download(["coord"])["coord"], before)
     g.run(5)
     assert g.scalars()[2] == 10
+
+
+def test_batch_reports_the_failing_model_only():
+    from dynearthsol_amd.lib import DesBatch, DesError
+    good, bad = SC.build("evp3"), copy.deepcopy(SC.build("elastic3"))
+    bad.params.rheol_type = 6                   # no such rheology
+    bt = DesBatch([good, bad])
+    bt.run(3)
+    assert bt.scalars(0)[2] == 3
+    with pytest.raises(DesError) as ex:
+        bt.scalars(1)
+    assert ex.value.code == abi.DES_ERR_INPUT
