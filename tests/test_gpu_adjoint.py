@@ -153,11 +153,17 @@ def test_taped_run_matches_checkpointed_run():
 
 
 def test_oversized_tape_is_refused():
+    """Refused before any step; a failed forward leaves nothing for a
+    backward to run on (status 101, call out of order)."""
     from dynearthsol_amd.lib import DesError, DesModel
-    m, _ = _case("maxwell")
+    m, cb = _case("maxwell")
     g = DesModel(m)
+    g.adj_forward(3, 2)
     with pytest.raises(DesError):
         g.adj_forward(2_000_000_000, 0)
+    with pytest.raises(DesError) as ex:
+        g.adj_backward(cb)
+    assert ex.value.code == 101
 
 
 def test_torch_autograd_binding():
