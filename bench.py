@@ -1,12 +1,17 @@
 """Benchmark of the explicit time step on one MI355X.
 
-Default workload (--workload 3d-fp64): the 3D FP64 extension box -- 100 x 100 x
-10 km, ~1.02 M tetrahedra / 187 k nodes, elasto-visco-plastic Mohr-Coulomb
-(cohesion 44 -> 4 MPa, friction 30 -> 5 deg over plastic strain 0.1), linear
-273 -> 1573 K temperature, 5 km weak slab plus seeded plastic-strain noise,
-+-1 cm/yr side walls, dynamic-relaxation mass scaling.
---workload 3d-mixed: 200 x 200 x 20 km, ~8.1 M tetrahedra / 1.4 M nodes,
-per-element log-viscosity U[21,25] and cohesion N(44, 4.4) MPa.
+Default workload (--workload 3d-8m): the 3D box of the mixed-precision
+configuration at full size -- 200 x 200 x 20 km, ~8.09 M tetrahedra / 1.42 M
+nodes, elasto-visco-plastic Mohr-Coulomb (cohesion softening 44 -> 4 MPa
+scaled per element, friction 30 -> 5 deg over plastic strain 0.1), per-element
+log10 viscosity U[21,25] and cohesion N(44, 4.4) MPa, linear 273 -> 1573 K
+temperature, 5 km weak slab, +-1 cm/yr side walls, dynamic-relaxation mass
+scaling -- stepped in FP64; the mixed-precision step of the same model runs
+beside it, and a short differentiated run of the adjoint configuration gives
+the adjoint/forward time.
+--workload 3d-fp64: 100 x 100 x 10 km, ~1.02 M tetrahedra (FP64).
+--workload 3d-mixed: the 8.09 M-tet model with mixed precision primary.
+--workload adjoint / ensemble: the differentiated run and the batched models.
 
 Synthetic inputs are built on the host and resident in HBM before the timed
 region.  A step is one pass of the explicit step over the whole mesh through
@@ -15,10 +20,16 @@ the fused six-pass schedule (des_run).  Prints one JSON line.
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 
-import numpy as np
+# the CPU baselines' OpenMP placement, read when the OpenMP runtime loads
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -29,35 +40,38 @@ METRIC = "elem-steps/s & % HBM peak: FP64 vs mixed-prec step; adjoint/forward ti
 # Algorithmic bytes of the fused passes per element and per node (3D, FP64):
 # pass C reads conn, volume, volume_old, stress, strain, plstrain and writes
 # stress, strain, plstrain, delta_plstrain, dpressure (256 B/element) and
-# gathers coordinates, velocity, dvoldt and temperature (64 B/node).
+# gathers coordinates, velocity, dvoldt and temperature (64 B/node).  Models
+# with per-element viscosity and cohesion read them in pass C (+16 B/element).
 PASS_BYTES_3D = {
     "pass_a": (72, 56), "pass_b": (72, 48), "pass_c": (256, 64),
     "pass_d": (32, 20), "pass_e": (200, 40), "pass_f": (112, 112),
 }
-# mixed precision: FP32 deviator/strain/plastic strain in passes C and E
-PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(152, 64), pass_e=(160, 40))
+# mixed precision: FP32 deviator, strain and rheology in passes C and E, FP64
+# plastic strain; force contributions in 12-byte FP32 slots (E writes, F reads)
+PASS_BYTES_3D_MIXED = dict(PASS_BYTES_3D, pass_c=(156, 64), pass_e=(112, 40), pass_f=(64, 112))
+ELEM_PARAM_BYTES = {"fp64": 16, "mixed": 8}     # per-element viscosity and cohesion, pass C
 
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per
-# dispatch, averaged over 600 dispatches), each scaled by what the counter
-# reads for a plain 1 GiB copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the
-# bytes moved): (2 FETCH + WRITE) * 1024.  Measured on the 3d-fp64 workload
-# (1019592 tetrahedra) and the 3d-mixed workload (8088336 tetrahedra, mixed
-# precision); the commands are in profiles/.
+# dispatch), each scaled by what the counter reads for a plain 1 GiB copy
+# (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved): (2 FETCH + WRITE)
+# * 1024.  The commands and per-kernel tables are in profiles/.
 TRAFFIC = {
     "3d-fp64": {"k_pass_a": 92690817, "k_pass_b": 225450346, "k_pass_c": 302206372, "k_pass_d": 35101817,
                 "k_pass_e": 239278641, "k_pass_f": 154240694},
-    "3d-mixed": {"k_pass_a": 736108774, "k_pass_b": 1952328345, "k_pass_c_mixed": 1627773140,
-                 "k_pass_d": 639150265, "k_pass_e_mixed": 1751009560, "k_pass_f": 1287281115},
 }
 
 WORKLOADS = {
+    "3d-8m": ("3d box 200x200x20 km, 8.09 M tets, per-element viscosity and cohesion, EVP, FP64 step "
+              "(mixed-precision step beside it)", 8_000_000),
     "3d-fp64": ("3d-fp64 extension box 100x100x10 km, EVP Mohr-Coulomb", 1_000_000),
-    "3d-mixed": ("3d-mixed box 200x200x20 km, per-element viscosity and cohesion, EVP", 8_000_000),
+    "3d-mixed": ("3d box 200x200x20 km, 8.09 M tets, per-element viscosity and cohesion, EVP, "
+                 "mixed-precision step", 8_000_000),
     "adjoint": ("adjoint box 50x50x10 km, maxwell, per-element log10 viscosity U[22,24]", None),
     "ensemble": ("ensemble of 256 box models (128 2D with 1k-50k triangles, 128 3D with 10k-200k "
                  "tetrahedra, log-uniform sizes, per-model extension rate, friction angle and "
                  "weak-seed position, seed 6), mixed precision", None),
 }
+MODEL_OF = {"3d-8m": "3d-mixed", "3d-fp64": "3d-fp64", "3d-mixed": "3d-mixed"}
 
 
 def kernel_name(precision, p):
@@ -68,27 +82,62 @@ def kernel_name(precision, p):
     return "k_dt_partial+k_dt_final" if p == "dt" else "k_" + p
 
 
-def cpu_baseline(model, nsteps):
-    """The reference's own step functions (oracle/_ref, OpenMP build) timed on
-    this host over a bounded sample of the same workload."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baselines(model, nsteps, runs=3):
+    """The CPU step on this host over a bounded sample of the same workload:
+    the reference's own step functions (oracle/_ref, OpenMP build; its
+    rheology.cxx is serial because its OpenMP clause does not compile) and
+    the oracle's C restatement (every function OpenMP-parallel).  Median of
+    `runs` runs of nsteps steps each, from the same initial state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
+    host = {"cpu": cpu_model(), "nproc": os.cpu_count(), **env}
     try:
         from oracle import OracleState, ref_available, ref_lib
     except Exception as e:   # the checker is absent: report why
-        return {"value": None, "unit": "elem-steps/s", "cores": 0, "kind": "reference",
-                "sample": f"unavailable: {e}"}
-    if not ref_available(3, omp=True):
-        return {"value": None, "unit": "elem-steps/s", "cores": 0, "kind": "reference",
-                "sample": "oracle/_ref not built"}
-    o = OracleState(model)
-    threads = ref_lib(3, omp=True).ref3_nthreads()
-    secs = o.ref_run(nsteps, omp=True)
-    return {"value": model.nelem * nsteps / secs, "unit": "elem-steps/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"{nsteps} steps of the same {model.nelem}-element model, reference "
-                      f"fields/geometry/bc/matprops with OpenMP ({threads} threads), "
-                      f"rheology.cxx serial (its OpenMP clause does not compile), "
-                      f"{secs:.2f} s"}
+        return ({"value": None, "unit": "elem-steps/s", "cores": 0, "kind": "reference",
+                 "sample": f"unavailable: {e}"}, None)
+    base = OracleState(model)
+    out = []
+    if ref_available(3, omp=True):
+        threads = ref_lib(3, omp=True).ref3_nthreads()
+        secs = [base.copy().ref_run(nsteps, omp=True) for _ in range(runs)]
+        med = statistics.median(secs)
+        out.append({"value": model.nelem * nsteps / med, "unit": "elem-steps/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"median of {runs} runs of {nsteps} steps of the same {model.nelem}-element "
+                              f"model from its initial state ({', '.join(f'{s:.2f}' for s in secs)} s): the "
+                              f"reference's fields/geometry/bc/matprops with OpenMP ({threads} threads), "
+                              f"rheology.cxx serial (its OpenMP clause does not compile)",
+                    "host": host})
+    else:
+        out.append({"value": None, "unit": "elem-steps/s", "cores": 0, "kind": "reference",
+                    "sample": "oracle/_ref not built"})
+    secs = []
+    for _ in range(runs):
+        o = base.copy()
+        t0 = time.perf_counter()
+        o.step(nsteps)
+        secs.append(time.perf_counter() - t0)
+    med = statistics.median(secs)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+    out.append({"value": model.nelem * nsteps / med, "unit": "elem-steps/s", "cores": threads,
+                "kind": "port",
+                "sample": f"median of {runs} runs of {nsteps} steps of the same model "
+                          f"({', '.join(f'{s:.2f}' for s in secs)} s): the oracle's C restatement of the "
+                          f"step, every function OpenMP-parallel ({threads} threads)",
+                "host": host})
+    return out[0], out[1]
 
 
 def timed(g, steps, precision):
@@ -124,6 +173,7 @@ def bench_ensemble(args):
     secs = time.perf_counter() - t0
     value = E * args.steps / secs
     print(f"# batch: {1e3 * secs / args.steps:.4f} ms per step of all models", file=sys.stderr, flush=True)
+    tuning = {nd: bt.tuning(nd)[0] for nd in (2, 3)}
     bt.close()
     # one at a time: every model, seq_steps timed steps each after a short warmup
     seq_secs, seq_work = 0.0, 0
@@ -140,9 +190,10 @@ def bench_ensemble(args):
         "metric": METRIC, "value": value, "unit": "elem-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 deviator/rheology + f64 pressure/geometry/force", "data": "synthetic",
+        "dtype": "f32 deviator/rheology + f64 pressure/geometry/force accumulation", "data": "synthetic",
         "config": {"workload": WORKLOADS["ensemble"][0], "models": len(models), "models_2d": n2,
-                   "nelem_total": E, "precision": "mixed", "schedule": "batched fused six-pass"},
+                   "nelem_total": E, "precision": "mixed", "schedule": "batched fused six-pass",
+                   "tuning": tuning},
         "one_at_a_time": {"value": seq_value, "steps_per_model": args.seq_steps,
                           "batch_speedup": value / seq_value},
     }
@@ -157,6 +208,8 @@ def adjoint_timing(nsteps, window, nelem, case="maxwell"):
     from dynearthsol_amd.lib import DesModel
     m = M.config_adjoint(case, nelem_target=nelem)
     g = DesModel(m)
+    g.run(20)          # first launches of every kernel, outside the timing
+    g.reset()
     t_plain = timed(g, nsteps, 0)
     g.reset()
     g.sync()
@@ -202,13 +255,15 @@ def bench_adjoint(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d-fp64")
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="3d-8m")
     ap.add_argument("--nelem", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps per CPU-baseline run (default: about 10 s of CPU work per run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-nelem", type=int, default=200_000)
     ap.add_argument("--ensemble-scale", type=float, default=1.0)
@@ -229,8 +284,36 @@ def main():
     desc, ne_default = WORKLOADS[args.workload]
     nelem = args.nelem or ne_default
     primary = abi.DES_MIXED if args.workload == "3d-mixed" else abi.DES_FP64
-    m = M.config_benchmark(args.workload, nelem_target=nelem)
+    t0 = time.perf_counter()
+    m = M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem)
     E, N = m.nelem, m.nnode
+    print(f"# {args.workload}: {E} elements, {N} nodes built in {time.perf_counter() - t0:.1f} s; "
+          f"warmup {args.warmup}", file=sys.stderr, flush=True)
+    has_params = m.elem_visc is not None
+
+    def pass_table(precision):
+        table = dict(PASS_BYTES_3D_MIXED if precision == abi.DES_MIXED else PASS_BYTES_3D)
+        if has_params:
+            be, bn = table["pass_c"]
+            table["pass_c"] = (be + ELEM_PARAM_BYTES["mixed" if precision == abi.DES_MIXED else "fp64"], bn)
+        return table
+
+    # The pass-C autotuner runs before any warm-up, on a throwaway model of
+    # the same mesh; the timed models get its choice.
+    tuning = {}
+
+    def tuned(precision):
+        if precision == abi.DES_MIXED:
+            return -1           # the mixed pass C has one variant
+        if precision not in tuning:
+            t = DesModel(m)
+            t.run(16, precision)
+            t.sync()
+            tuning[precision] = t.tuning()
+            t.close()
+            assert tuning[precision][0] >= 0, tuning
+        return tuning[precision][0]
+
     # Two identical models per precision.  g's warm-up and the twin's warm-up
     # and timed steps run under per-pass events that the dispatches timestamp
     # (kernel execution time); g's timed steps run plain (the timing) and see
@@ -247,12 +330,12 @@ def main():
             t[0] += times * ms
             t[1] += times * c
 
-    def timed_pair(precision, tuning=None):
+    def timed_pair(precision):
+        variant = tuned(precision)
         a, b = DesModel(m), DesModel(m)
-        if tuning is not None:
-            a.set_tuning(tuning)
+        a.set_tuning(variant)
+        b.set_tuning(variant)
         wa = a.run_timed(args.warmup, precision) if args.warmup else {}
-        b.set_tuning(a.tuning()[0])
         wb = b.run_timed(args.warmup, precision) if args.warmup else {}
         a.sync()
         b.sync()
@@ -265,13 +348,12 @@ def main():
         b.close()
         return a, secs, passes
 
-    print(f"# {args.workload}: {E} elements, {N} nodes; warmup {args.warmup}", file=sys.stderr, flush=True)
     g, secs, passes = timed_pair(primary)
     ms_per_step = 1e3 * secs / args.steps
     value = E * args.steps / secs
     print(f"# timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
     avg = {k: ms / max(c, 1) for k, (ms, c) in passes.items()}
-    table = PASS_BYTES_3D_MIXED if primary == abi.DES_MIXED else PASS_BYTES_3D
+    table = pass_table(primary)
     dom = max((k for k in avg if k in table), key=lambda k: avg[k])
     be, bn = table[dom]
     step_bytes = sum(be_ * E + bn_ * N for be_, bn_ in table.values())
@@ -282,15 +364,20 @@ def main():
     compare = None
     if not args.no_mixed:
         other = abi.DES_FP64 if primary == abi.DES_MIXED else abi.DES_MIXED
-        g2, t2, _ = timed_pair(other)
-        names = ["vel", "stress", "coord", "plstrain"]
+        g2, t2, passes2 = timed_pair(other)
+        names = ["vel", "stress", "coord", "plstrain", "temperature"]
         a, b = g2.download(names), g.download(names)
         t64, t32 = (secs, t2) if primary == abi.DES_FP64 else (t2, secs)
         if primary != abi.DES_FP64:
             a, b = b, a
+        mixed_passes = passes2 if primary == abi.DES_FP64 else passes
+        mt = pass_table(abi.DES_MIXED)
+        mixed_bytes = sum(be_ * E + bn_ * N for be_, bn_ in mt.values())
         compare = {"steps": args.warmup + args.steps,
                    "fp64_ms_per_step": 1e3 * t64 / args.steps, "mixed_ms_per_step": 1e3 * t32 / args.steps,
                    "mixed_speedup": t64 / t32,
+                   "mixed_speedup_bound_by_bytes": step_bytes / mixed_bytes if primary == abi.DES_FP64 else None,
+                   "mixed_pass_ms": {k: round(ms / max(c, 1), 5) for k, (ms, c) in mixed_passes.items()},
                    "mixed_rel_err_vs_fp64": {k: rel(a[k], b[k]) for k in names}}
         g2.close()
     g.close()
@@ -298,9 +385,17 @@ def main():
     proc_avg = tot_ms / tot_n
     achieved = (be * E + bn * N) / (proc_avg * 1e-3) / 1e9
 
-    cpu = None
+    adjoint = None
+    if not args.no_adjoint and args.workload == "3d-8m":
+        adj = adjoint_timing(200, 100, args.adjoint_nelem)
+        adjoint = {"gradient_over_forward": adj["gradient_over_forward"], **adj}
+        print(f"# adjoint/forward {adj['gradient_over_forward']:.3f}x", file=sys.stderr, flush=True)
+
+    cpu = port = None
     if not args.no_cpu:
-        cpu = cpu_baseline(M.config_benchmark(args.workload, nelem_target=nelem), args.cpu_steps)
+        # about 10 s of CPU work per run at the reference's measured ~3 M elem-steps/s on 16 cores
+        steps = args.cpu_steps or max(1, int(3e7 / E))
+        cpu, port = cpu_baselines(M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem), steps)
 
     out = {
         "metric": METRIC,
@@ -313,10 +408,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if primary == abi.DES_FP64 else "f32 deviator/rheology + f64 pressure/geometry/force",
+        "dtype": "f64" if primary == abi.DES_FP64 else "f32 deviator/rheology + f64 pressure/geometry/force accumulation",
         "data": "synthetic",
         "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
-                   "precision": "fp64" if primary == abi.DES_FP64 else "mixed"},
+                   "precision": "fp64" if primary == abi.DES_FP64 else "mixed",
+                   "tuning": {("fp64" if p == abi.DES_FP64 else "mixed"): v for p, v in tuning.items()}},
         "roofline": {"bound": "hbm", "kernel": kernel, "pass": dom, "avg_ms": proc_avg,
                      "launches": tot_n, "timed_window_avg_ms": avg[dom],
                      "bytes_per_launch": be * E + bn * N,
@@ -327,7 +423,10 @@ def main():
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},
         "fp64_vs_mixed": compare,
+        "adjoint_over_forward": adjoint["gradient_over_forward"] if adjoint else None,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
+        "cpu_port": port,
     }
     print(json.dumps(out), flush=True)
 

@@ -34,9 +34,11 @@ class _Run(torch.autograd.Function):
         g.reset()
         g.set_elem_params(elem_visc=visc, elem_cohesion=coh)
         g.adj_forward(topo.nsteps, topo.window)
+        topo.runs += 1
         z = g.download(["coord"])["coord"][topo.top, -1]
         uz = z - topo.z0
         ctx.topo = topo
+        ctx.run = topo.runs
         ctx.visc = visc
         ctx.device = log_visc.device
         return torch.as_tensor(uz, dtype=torch.float64, device=log_visc.device)
@@ -44,6 +46,11 @@ class _Run(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_uz):
         topo = ctx.topo
+        if ctx.run != topo.runs:
+            # the device holds the checkpoints (or tape) of the latest forward only
+            raise RuntimeError("SurfaceTopography: another forward ran after this graph's; its device "
+                               "checkpoints are gone (run the backward before the next forward, or use "
+                               "one SurfaceTopography per graph)")
         cb = np.zeros((topo.g.N, topo.g.nd))
         cb[topo.top, -1] = grad_uz.detach().double().cpu().numpy()
         vb, hb = topo.g.adj_backward(cb)
@@ -63,6 +70,7 @@ class SurfaceTopography:
         self.window = window
         self.top = np.nonzero(model.bcflag & BOUNDZ1)[0]
         self.z0 = model.coord[self.top, -1].copy()
+        self.runs = 0          # forwards so far; a graph's backward needs the latest one
 
     def __call__(self, log_visc, cohesion):
         return _Run.apply(log_visc, cohesion, self)

@@ -455,6 +455,21 @@ template <int ND> __device__ __forceinline__ void store_force_contrib(const DevS
     }
 }
 
+// The mixed-precision schedule's contributions: the same slots holding FP32
+// values (12 bytes in 3D), summed in FP64 by the node pass.
+template <int ND> __device__ __forceinline__ void store_force_contrib_f32(const DevState &s, int e,
+                                                                        const double out[Dim<ND>::NPE][ND])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    float *fw = reinterpret_cast<float *>(s.fwork);
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        float *w = fw + (long)s.slot[e * NPE + i] * ND;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) w[d] = (float)out[i][d];
+    }
+}
+
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_elem(DevState s)
 {
     BIND_ELEM(s, e);
@@ -559,13 +574,34 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevS
 // node part of update_force: gather, stress BCs, elastic foundation,
 // damping (fields.cxx:291-340, bc.cxx:493-566, fields.cxx:235-288)
 template <int ND> __device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND],
-                                                           const double v[ND])
+                                                           const double v[ND], bool f32_slots = false)
 {
     constexpr int NPE = Dim<ND>::NPE;
     const des_params *P = s.P;
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[d] = 0;
-    {
+    if (f32_slots) {
+        // FP32 contributions (mixed precision), accumulated in FP64 in slot order
+        const float *fw = reinterpret_cast<const float *>(s.fwork);
+        const int p1 = s.csr_off[n + 1];
+        int p = s.csr_off[n];
+        for (; p + 4 <= p1; p += 4) {
+            float c[4][ND];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) c[j][d] = fw[(long)(p + j) * ND + d];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) f[d] -= (double)c[j][d];
+        }
+        for (; p < p1; ++p) {
+            const float *c = fw + (long)p * ND;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) f[d] -= (double)c[d];
+        }
+    } else {
         const int p1 = s.csr_off[n + 1];
         int p = s.csr_off[n];
         for (; p + 4 <= p1; p += 4) {   // four entries in flight, summed in order
@@ -882,14 +918,29 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial(DevState
     if (threadIdx.x < 3) s.red[blockIdx.x * 3 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb, double bulkm0, double rho0)
+// bulkm(0) and rho(0) of the dynamic elastic dt (geometry.cxx:297): element 0
+// is the reference's (host) element 0, wherever the device numbering put it
+template <int ND> __device__ __forceinline__ void elem0_props(const DevState &s, int e0, double &bulkm0,
+                                                            double &rho0)
+{
+    int c[Dim<ND>::NPE];
+    load_conn<ND>(s, e0, c);
+    bulkm0 = harmonic_mean(s, s.P->bulk_modulus, e0);
+    rho0 = mat_rho(s, e0, elem_mean<ND>(s.T, c));
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb)
 {
     double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     for (int b = threadIdx.x; b < nb; b += BLOCK)
         for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
     __shared__ double sm[3][BLOCK];
     block_min3(sm, v);
-    if (threadIdx.x == 0) store_dt(s.P, s.sc, sm[0][0], sm[1][0], sm[2][0], bulkm0, rho0);
+    if (threadIdx.x == 0) {
+        double bulkm0 = 0, rho0 = 1;
+        if (!s.P->is_quasi_static && s.P->fixed_dt == 0) elem0_props<ND>(s, s.e0dev[0], bulkm0, rho0);
+        store_dt(s.P, s.sc, sm[0][0], sm[1][0], sm[2][0], bulkm0, rho0);
+    }
 }
 
 // compute_dt of batched models: element block b holds elements of one model
@@ -918,23 +969,10 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final_b(DevState
     __shared__ double sm[3][BLOCK];
     block_min3(sm, v);
     if (threadIdx.x == 0) {
-        const int e0 = b0 * BLOCK;
-        int c[Dim<ND>::NPE];
-        load_conn<ND>(s, e0, c);
-        const double bulkm0 = harmonic_mean(s, s.P->bulk_modulus, e0);
-        const double rho0 = mat_rho(s, e0, elem_mean<ND>(s.T, c));
+        double bulkm0 = 0, rho0 = 1;
+        if (!s.P->is_quasi_static && s.P->fixed_dt == 0) elem0_props<ND>(s, s.e0dev[m], bulkm0, rho0);
         store_dt(s.P, s.sc, sm[0][0], sm[1][0], sm[2][0], bulkm0, rho0);
     }
-}
-
-// rho(0) and bulkm(0) for the dynamic elastic dt (geometry.cxx:297)
-template <int ND> __global__ void k_elem0_props(DevState s, double *out)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int c[Dim<ND>::NPE];
-    load_conn<ND>(s, 0, c);
-    out[0] = harmonic_mean(s, s.P->bulk_modulus, 0);
-    out[1] = mat_rho(s, 0, elem_mean<ND>(s.T, c));
 }
 
 // ---- worst_elem_quality (geometry.cxx:458-512)
@@ -1113,6 +1151,7 @@ struct FusedFlags {
     int use_stored_mass = 0;   // first step after a sync: mass/tmass/volume_n from compute_mass
     int store_aux = 0;         // last step before a sync: keep strain_rate, edvoldt, force
     int step_end = 0;          // pass F also ends the step: time, steps, dt_prev, rotation flag
+    int f32_slots = 0;         // force contributions in FP32 slots (mixed precision)
     // the differentiated run's step records, written by the pass that has
     // the value at hand (null in plain runs): pass C (FP64) the stress,
     // plastic strain and volume_old it starts from, pass B the node records
@@ -1458,7 +1497,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
     double f[ND], v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = __builtin_nontemporal_load(s.vel + d * s.N + n);
-    node_force<ND>(s, n, f, v);
+    node_force<ND>(s, n, f, v, fl.f32_slots);
     if (fl.store_aux) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
@@ -1501,7 +1540,6 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_to_mixed(DevState s
         s.sdev[k * E + e] = sd[k];
         s.strain32[k * E + e] = (float)s.strain[k * E + e];
     }
-    s.pl32[e] = (float)s.plstrain[e];
     s.dpl32[e] = (float)s.delta_plstrain[e];
     if (s.visc32) const_cast<float *>(s.visc32)[e] = (float)s.elem_visc[e];
     if (s.coh32) const_cast<float *>(s.coh32)[e] = (float)s.elem_cohesion[e];
@@ -1520,7 +1558,6 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState
     store_tensor<ND>(s.stress, E, e, st);
 #pragma unroll
     for (int k = 0; k < NSTR; ++k) s.strain[k * E + e] = (double)s.strain32[k * E + e];
-    s.plstrain[e] = (double)s.pl32[e];
     s.delta_plstrain[e] = (double)s.dpl32[e];
 }
 
@@ -1567,7 +1604,8 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
 #pragma unroll
     for (int i = 0; i < NPE; ++i) edv += aux[i];
     edv = edv / NPE;
-    float pl = __builtin_nontemporal_load(s.pl32 + e), dpl = __builtin_nontemporal_load(s.dpl32 + e);
+    double pl = __builtin_nontemporal_load(s.plstrain + e);
+    float dpl = __builtin_nontemporal_load(s.dpl32 + e);
     double dp;
     const float Tm = (float)node_mean<ND>(Tn);
     bool deferred;
@@ -1581,7 +1619,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
         s.pres[e] = p;
 #pragma unroll
         for (int k = 0; k < NSTR; ++k) s.sdev[k * E + e] = sd[k];
-        __builtin_nontemporal_store(pl, s.pl32 + e);
+        __builtin_nontemporal_store(pl, s.plstrain + e);
         __builtin_nontemporal_store(dpl, s.dpl32 + e);
         s.dpressure[e] = dp;
         s.escal[e] = dp * vol;
@@ -1623,9 +1661,10 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
         const MixedRet &r = s.ret_mix[k];
         const float K = (float)harmonic_mean(st, P->bulk_modulus, e);
         const float G = (float)harmonic_mean(st, P->shear_modulus, e);
-        float pl = s.pl32[e], dpl = s.dpl32[e];
+        double pl = s.plstrain[e];
+        float dpl = s.dpl32[e];
         double amc, anphi, anpsi, hardn, ten_max;
-        mat_plastic_props_f(st, e, pl, amc, anphi, anpsi, hardn, ten_max);
+        mat_plastic_props_f(st, e, (float)pl, amc, anphi, anpsi, hardn, ten_max);
         double s64[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) s64[i] = r.s64[i];
@@ -1646,13 +1685,13 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
             dpl = 0;
         } else {
             split_stress<3>(s64, p, sd);
-            pl += (float)depls;
+            pl += depls;
             dpl = (float)depls;
         }
         s.pres[e] = p;
 #pragma unroll
         for (int i = 0; i < 6; ++i) s.sdev[i * E + e] = sd[i];
-        s.pl32[e] = pl;
+        s.plstrain[e] = pl;
         s.dpl32[e] = dpl;
         const double dp = 3 * (p - r.p_old);
         s.dpressure[e] = dp;
@@ -1679,7 +1718,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevSta
     const double vol = s.volume[e];
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
-    store_force_contrib<ND>(s, e, out);
+    store_force_contrib_f32<ND>(s, e, out);
 }
 
 // node records from the SoA state, at the start of a fused run
@@ -1735,6 +1774,7 @@ struct AdjCkpt {
 struct des_ctx {
     des_params P;
     int nd, npe, nstr, N, E, nbf;
+    int device = 0;                // the HIP device current at des_create
     hipStream_t stream = nullptr;
     // start/stop events handed to the next launches (timed runs only): the
     // dispatch itself timestamps them, so they time the kernel's execution
@@ -1746,7 +1786,6 @@ struct des_ctx {
     int vol_cur = 0;               // s.volume == vol_buf[vol_cur]
     int *active_facets = nullptr;  // facets whose traction enters the force
     int nactive = 0;
-    double *elem0 = nullptr;       // bulkm(0), rho(0)
     int steps = 0;                 // host mirror of var.steps
     int avg_interval = 0;          // Output::average_interval (0: averaged fields off)
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
@@ -1950,17 +1989,8 @@ int launch_compute_dt(des_ctx *ctx)
         return DES_OK;
     }
     if (ctx->P.fixed_dt == 0) LAUNCH_ND(k_dt_partial, RED_BLOCKS, ctx->s);
-    double bulkm0 = 0, rho0 = 1;
-    if (!ctx->P.is_quasi_static && ctx->P.fixed_dt == 0) {
-        LAUNCH_ND(k_elem0_props, 1, ctx->s, ctx->elem0);
-        double h[2];
-        HIPCHK(hipMemcpyAsync(h, ctx->elem0, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        bulkm0 = h[0];
-        rho0 = h[1];
-    }
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLOCK), 0, ctx->stream, ctx->s, RED_BLOCKS, bulkm0, rho0);
-    return check_launch(ctx);
+    LAUNCH_ND(k_dt_final, 1, ctx->s, RED_BLOCKS);
+    return DES_OK;
 }
 
 int launch_mass(des_ctx *ctx)
@@ -2142,12 +2172,36 @@ void order_mesh(const des_mesh &in, int nd, int nmat, int mode, OrderedMesh &o)
         for (long i = 0; i < N; ++i) o.nh[i] = (int)i;
         for (long i = 0; i < E; ++i) o.eh[i] = (int)i;
     } else {
-        curve_order(in.coord, N, nd, o.nh);
         std::vector<double> cen((size_t)E * nd, 0.0);
         for (long e = 0; e < E; ++e)
             for (int i = 0; i < npe; ++i)
                 for (int d = 0; d < nd; ++d) cen[e * nd + d] += in.coord[(size_t)in.connectivity[e * npe + i] * nd + d];
         curve_order(cen.data(), E, nd, o.eh);
+        if (mode == 1) {
+            curve_order(in.coord, N, nd, o.nh);
+        } else {
+            // nodes follow the elements: numbered in the order the curve-ordered
+            // elements first touch them (mode 2), or by the mean device index
+            // of their elements (mode 3), so that a block of nodes gathers from
+            // one compact window of element records
+            std::vector<long> key(N, -1), sum(N, 0);
+            std::vector<int> cnt(N, 0);
+            long next = 0;
+            for (long e = 0; e < E; ++e)
+                for (int i = 0; i < npe; ++i) {
+                    const int n = in.connectivity[(size_t)o.eh[e] * npe + i];
+                    if (key[n] < 0) key[n] = next++;
+                    sum[n] += e;
+                    cnt[n]++;
+                }
+            std::vector<std::pair<double, long>> k2(N);
+            for (long n = 0; n < N; ++n) {
+                const long first = key[n] < 0 ? next + n : key[n];      // unused nodes last
+                k2[n] = {mode == 2 || !cnt[n] ? (double)first : (double)sum[n] / cnt[n], first};
+            }
+            for (long n = 0; n < N; ++n) o.nh[n] = (int)n;
+            std::sort(o.nh.begin(), o.nh.end(), [&](int a, int b) { return k2[a] < k2[b]; });
+        }
     }
     std::vector<int> ninv(N), einv(E);
     for (long i = 0; i < N; ++i) ninv[o.nh[i]] = (int)i;
@@ -2191,6 +2245,7 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
     ctx->nbf = m->nbfacet;
     const int nmodels = g ? (int)g->P.size() : 0;
     ctx->nmodels = nmodels;
+    HIPCHK(hipGetDevice(&ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     DevState &s = ctx->s;
     s.N = N;
@@ -2435,7 +2490,20 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
     const int nred = std::max(RED_BLOCKS, nblocks(E));
     ALLOC(s.red, (size_t)nred * 3);
     ALLOC(s.redi, (size_t)nred * 2);
-    ALLOC(ctx->elem0, 2);
+    {
+        // the device index of each model's host element 0 (compute_dt's
+        // bulkm(0) and rho(0), geometry.cxx:297)
+        std::vector<int> e0(std::max(nmodels, 1), 0);
+        for (int k = 0; k < (int)e0.size(); ++k) {
+            const int lo = nmodels ? g->e_off[k] : 0, hi = nmodels ? lo + g->e_cnt[k] : E;
+            for (int e = lo; e < hi; ++e)
+                if (eh[e] == 0) { e0[k] = e; break; }
+        }
+        int *d_e0;
+        ALLOC(d_e0, e0.size());
+        H2D(d_e0, e0.data(), e0.size());
+        s.e0dev = d_e0;
+    }
     if (nmodels) {
         // block -> model tables; a block's real elements (nodes) end at lim
         const int neb = nblocks(E), nnb = nblocks(N);
@@ -2521,7 +2589,7 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
     return DES_OK;
 }
 
-void des_set_mesh_order(int mode) { g_mesh_order = mode ? 1 : 0; }
+void des_set_mesh_order(int mode) { g_mesh_order = (mode >= 0 && mode <= 3) ? mode : 1; }
 
 void des_destroy(des_ctx *ctx)
 {
@@ -3085,7 +3153,6 @@ int ensure_mixed(des_ctx *ctx)
     if ((rc = dalloc_n(ctx, &s.pres, E))) return rc;
     if ((rc = dalloc_n(ctx, &s.sdev, E * nstr))) return rc;
     if ((rc = dalloc_n(ctx, &s.strain32, E * nstr))) return rc;
-    if ((rc = dalloc_n(ctx, &s.pl32, E))) return rc;
     if ((rc = dalloc_n(ctx, &s.dpl32, E))) return rc;
     if (ctx->nd == 3 && (rc = dalloc_n(ctx, &s.ret_mix, E))) return rc;
     if (s.elem_visc) {
@@ -3250,6 +3317,7 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     ARM(false);
     FusedFlags ff = fl;
     ff.step_end = 1;
+    ff.f32_slots = mixed;
     LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, ff);
     MARK(6);
     ctx->steps++;

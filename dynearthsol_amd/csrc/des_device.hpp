@@ -85,7 +85,7 @@ struct DevState {
     double *pres;                                 // E, mean of the stress diagonal
     float *sdev;                                  // NSTR x E, stress minus pres * I
     float *strain32;                              // NSTR x E
-    float *pl32, *dpl32;                          // E
+    float *dpl32;                                 // E (the plastic strain itself stays FP64)
     const float *visc32, *coh32;                  // E or null
     // batched models (null for a single model): elements and nodes of model m
     // fill whole blocks; block b of an element (node) kernel belongs to model
@@ -97,6 +97,7 @@ struct DevState {
     const double *bnb, *evb;                      // per-model boundary normals / edge vectors
     const int *evvb;                              // per-model edge_valid
     const int *mblk_off;                          // first element block of each model (+ total)
+    const int *e0dev;                             // per model: device index of host element 0
     int nmodels;
     // elements whose Mohr-Coulomb return is left to the return kernel of the
     // fused schedule: count, element ids, and per entry the trial stress and
@@ -573,6 +574,15 @@ __device__ __forceinline__ void mat_plastic_props(const DevState &s, int e, doub
 }
 
 // --------------------------------------------------------------- 3x3-C
+// The four routines below restate, step for step, Joachim Kopp's "Numerical
+// diagonalization of 3x3 matrices" (3x3-C: dsyevc3, dsytrd3, dsyevq3,
+// dsyevh3), Copyright (C) 2006 Joachim Kopp, distributed under the GNU
+// Lesser General Public License, version 2.1 or later (3x3-C/COPYING of the
+// reference).  The return mapping depends on that code's exact eigenvalue and
+// eigenvector choices, so its steps are kept; this restatement is covered by
+// the same licence.  Published description: J. Kopp, "Efficient numerical
+// diagonalization of hermitian 3x3 matrices", Int. J. Mod. Phys. C 19 (2008)
+// 523-548.
 #define DES_SQR(x) ((x) * (x))
 
 // Cardano eigenvalues (3x3-C/dsyevc3.c:31-81); A given by its upper triangle
@@ -1349,7 +1359,7 @@ template <int ND, bool DEFER = false>
 __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf, float edvoldt,
                                                     float Tmean, double vol, double vol_old, double &p,
                                                     float *sd, const float *rs, float *es, const float *re,
-                                                    float *edot, float &pl, float &dpl, double &dpressure,
+                                                    float *edot, double &pl, float &dpl, double &dpressure,
                                                     bool &deferred, MixedRet &rec)
 {
     deferred = false;
@@ -1410,7 +1420,7 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
             svII = second_invariant2_f<ND>(sv);
         }
         double amc, anphi, anpsi, hardn, ten_max;
-        mat_plastic_props_f(st, e, pl, amc, anphi, anpsi, hardn, ten_max);
+        mat_plastic_props_f(st, e, (float)pl, amc, anphi, anpsi, hardn, ten_max);
         double pp = p;
         elastic_incr<ND>(K, G, de, rs, pp, inc);
         double s64[NSTR];
@@ -1451,7 +1461,7 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
             dpl = 0;
         } else {
             split_stress<ND>(s64, p, sd);
-            pl += (float)depls;
+            pl += depls;                 // accumulated in FP64: increments below an FP32 ulp count
             dpl = (float)depls;
         }
         break;

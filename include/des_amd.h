@@ -153,9 +153,12 @@ void des_destroy(des_ctx *ctx);
 /* Device numbering of the meshes given to later des_create / des_batch_create
    calls (process-wide; no counterpart in the reference): 1 (default) stores
    elements and nodes in Hilbert-curve order of their positions, 0 in the host's
-   order.  Every array crossing this interface stays in the host's numbering
-   and every index returned is a host index; each node sums its elements in
-   ascending host order, so results do not depend on the setting. */
+   order; 2 and 3 keep the elements' curve order and number the nodes by the
+   order in which those elements first touch them (2) or by the mean device
+   index of their elements (3).  Every array crossing this interface stays in
+   the host's numbering and every index returned is a host index; each node
+   sums its elements in ascending host order, so results do not depend on the
+   setting. */
 void des_set_mesh_order(int mode);
 const char *des_last_error(const des_ctx *ctx);
 

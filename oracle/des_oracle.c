@@ -1067,6 +1067,12 @@ void FN(apply_vbcs)(ostate *S, const des_params *P)       /* bc.cxx:179-490 */
 }
 
 /* ------------------------------------------------------------------- 3x3-C */
+/* The routines of this section restate, step for step, Joachim Kopp's
+ * "Numerical diagonalization of 3x3 matrices" (3x3-C: dsyevc3, dsytrd3,
+ * dsyevq3, dsyevh3), Copyright (C) 2006 Joachim Kopp, distributed under the
+ * GNU Lesser General Public License, version 2.1 or later (3x3-C/COPYING of
+ * the reference); this restatement is covered by the same licence.
+ * J. Kopp, Int. J. Mod. Phys. C 19 (2008) 523-548. */
 #if NDIMS == 3
 #define SQR(x) ((x) * (x))
 

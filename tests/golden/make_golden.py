@@ -17,14 +17,19 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
 import scenarios as SC  # noqa: E402
 from oracle import OracleState  # noqa: E402
 
-CASES = [("evp3", 500, 20), ("evp2", 400, 20), ("mat3", 500, 15), ("bc2", 400, 20),
-         ("psevp2", 400, 20), ("maxwell3", 500, 15)]
+# (scenario, elements, steps recorded, reference steps taken before the
+# recorded input state)
+CASES = [("evp3", 500, 20, 0), ("evp2", 400, 20, 0), ("mat3", 500, 15, 0), ("bc2", 400, 20, 0),
+         ("psevp2", 400, 20, 0), ("maxwell3", 500, 15, 0), ("viscous3", 500, 15, 0),
+         ("viscous2", 400, 20, 0), ("dynmat3", 500, 1, 9), ("bc3", 500, 1, 9)]
 
 
 def main():
-    for name, ne, nsteps in CASES:
+    for name, ne, nsteps, pre in CASES:
         m = SC.build(name, nelem=ne)
         o = OracleState(m)
+        if pre:
+            o.ref_run(pre, strict=True)
         out = {
             "nsteps": np.array(nsteps),
             "params": np.frombuffer(bytes(m.params), dtype=np.uint8).copy(),
@@ -43,7 +48,7 @@ def main():
         for k, v in o.f.items():
             out["out_" + k] = v.copy()
         out["scalars_out"] = np.array([o.S.dt, o.S.time, o.S.compensation_pressure, o.S.steps])
-        path = os.path.join(HERE, f"{name}.npz")
+        path = os.path.join(HERE, f"{name}_from{pre}.npz" if pre else f"{name}.npz")
         np.savez_compressed(path, **out)
         print(path, os.path.getsize(path))
 

@@ -54,6 +54,25 @@ def _layers(m):
     return em
 
 
+def _slant(m, shear):
+    """Shear the box (x -> x - shear * z, z <= 0) so that its x walls become
+    slanted planes, and flag them as the slant boundaries n0 (the former x1
+    wall) and n1 (the former x0 wall) with their normals and the edge vectors
+    of their intersections (bc.cxx:87-176); the velocity BCs then take the
+    slant branch of apply_vbcs (bc.cxx:350-436)."""
+    nd = m.ndims
+    m.coord = m.coord.copy()
+    m.coord[:, 0] -= shear * m.coord[:, nd - 1]
+    f = m.bcflag.copy()
+    x0, x1 = (f & M.BOUNDX0) != 0, (f & M.BOUNDX1) != 0
+    f &= ~np.uint32(M.BOUNDX0 | M.BOUNDX1)
+    f[x1] |= np.uint32(1 << 6)
+    f[x0] |= np.uint32(1 << 7)
+    m.bcflag = f
+    m.bfacets = M.boundary_facets(m.conn, m.bcflag, nd)
+    m.bnormals, m.edge_vectors, m.edge_valid = M.boundary_normals(m.coord, m.conn, m.bfacets, nd)
+
+
 def build(name, nelem=None, **mesh_kw):
     cfg = dict(BASE)
     kw = {}
@@ -94,6 +113,11 @@ def build(name, nelem=None, **mesh_kw):
         cfg["mat.rheology_type"] = "viscous"
         cfg["mat.min_viscosity"] = 1e20
         cfg["mat.max_viscosity"] = 1e22
+        # explicit viscous stepping is stable for eta below ~2 h K / (v s)
+        # (h element size, K bulk modulus, v the boundary speed, s the
+        # inertial scaling): 6e20 Pa s for the 3D mesh, 3e20 for the 2D one
+        cfg["mat.min_viscosity"] = 1e19
+        cfg["mat.max_viscosity"] = 2e20 if nd == 3 else 1e20
         perturb = 0
     elif name.startswith("bc"):
         # Winkler bottom, water-loaded top, lithostatic and free-slip side walls,
@@ -111,8 +135,12 @@ def build(name, nelem=None, **mesh_kw):
         if nd == 3:
             cfg.update({"bc.vbc_y0": 5, "bc.vbc_val_y0": V1, "bc.vbc_y1": 7, "bc.vbc_val_y1": 0.0,
                         "bc.vbc_x0": 4, "bc.vbc_val_x0": 0.5 * V1})
-    elif name.startswith("mat"):
-        # two materials, PREM reference pressure for initial stress
+    elif name.startswith("mat") or name.startswith("dynmat"):
+        # two materials, PREM reference pressure for initial stress; dynmat:
+        # the same without quasi-static mass scaling (compute_dt's elastic
+        # term then reads bulkm and rho of element 0)
+        if name.startswith("dynmat"):
+            cfg.update({"control.is_quasi_static": False, "control.damping_option": 0})
         cfg.update({"mat.num_materials": 2, "mat.rho0": [2700.0, 3300.0],
                     "mat.bulk_modulus": [40e9, 80e9], "mat.shear_modulus": [25e9, 45e9],
                     "mat.cohesion0": [10e6, 20e6], "mat.cohesion1": [1e6, 2e6],
@@ -136,9 +164,20 @@ def build(name, nelem=None, **mesh_kw):
         kw["plstrain_noise"] = (0.02, 1)
     elif name.startswith("dyn"):
         cfg.update({"control.is_quasi_static": False, "control.damping_option": 0})
+    elif name.startswith("slant"):
+        # (visco-elastic, so that trajectories do not amplify rounding) slant
+        # x walls: n0 (outer, former x1) pulled out along its normal at 1 cm/yr, n1 (former x0) held by the horizontal-normal rule (type
+        # 11; slanttb: v along the normal, type 3, and horizontal, type 13);
+        # free-slip y walls and bottom meet them at edges
+        t0, t1 = (3, 13) if name.startswith("slanttb") else (1, 11)
+        cfg.update({"mat.rheology_type": "maxwell", "mat.min_viscosity": 1e19, "mat.max_viscosity": 1e25,
+                    "bc.vbc_x0": 0, "bc.vbc_x1": 0, "bc.vbc_n0": t0, "bc.vbc_val_n0": V1,
+                    "bc.vbc_n1": t1, "bc.vbc_val_n1": V1, "bc.vbc_z0": 1, "bc.vbc_val_z0": 0.0})
     else:
         raise ValueError(name)
     m = M.Model(cfg, nd, nelem_target=ne, **kw, **mesh_kw)
+    if name.startswith("slant"):
+        _slant(m, 0.3)
     if perturb:
         _perturb_stress(m, perturb, 11)
     return m
@@ -146,7 +185,8 @@ def build(name, nelem=None, **mesh_kw):
 
 SCENARIOS = ["evp3", "evp2", "ep3", "ps2", "psevp2", "maxwell3", "maxwell2", "elastic3",
              "elastic2", "viscous3", "viscous2", "bc3", "bc2", "damp23", "damp22", "mat3",
-             "mat2", "fields3", "dyn3"]
+             "mat2", "fields3", "dyn3", "dyn2", "dynmat3", "dynmat2", "slant3", "slant2", "slanttb3",
+             "slanttb2"]
 
 
 def rel_err(a, b):

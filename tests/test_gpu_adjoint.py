@@ -195,3 +195,22 @@ def test_unsupported_configuration_is_refused():
     g = DesModel(SC.build("evp3"))
     with pytest.raises(DesError):
         g.adj_forward(2, 1)
+
+
+def test_torch_binding_refuses_a_stale_graph():
+    """The device keeps the checkpoints of the latest forward only: a graph
+    whose forward was followed by another forward cannot be differentiated
+    (it would return the later run's gradient), and says so."""
+    import torch
+    from dynearthsol_amd.autograd import SurfaceTopography
+    m, _ = _case("maxwell")
+    topo = SurfaceTopography(m, nsteps=6, window=3)
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    lv = torch.tensor(np.log10(m.elem_visc), dtype=torch.float64, device=dev, requires_grad=True)
+    coh = torch.tensor(m.elem_cohesion, dtype=torch.float64, device=dev, requires_grad=True)
+    first = topo(lv, coh).sum()
+    second = topo(lv * 1.001, coh).sum()
+    with pytest.raises(RuntimeError, match="another forward"):
+        first.backward()
+    second.backward()          # the latest graph still works
+    assert torch.isfinite(lv.grad).all()

@@ -29,7 +29,7 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("name", ["evp3", "evp2", "mat3", "mat2", "bc3", "bc2", "damp23", "fields3",
-                                  "elastic3", "psevp2"])
+                                  "elastic3", "psevp2", "dyn3", "dyn2", "dynmat3", "dynmat2", "viscous3"])
 def test_fused_run_is_independent_of_numbering(name):
     m = SC.build(name)
     print(name, m.nelem, "elements", flush=True)

@@ -403,3 +403,21 @@ def test_vbc_slant_boundary_sets_normal_velocity():
     o = _vbc_state(3, [M.BOUNDX0 | (1 << 6)], types_, vals, bnormals=bn)
     o.call("apply_vbcs")
     assert o.S.status == abi.DES_ERR_ASSERT
+
+
+@pytest.mark.parametrize("name,amplifies", [("dyn2", True), ("dynmat2", True), ("dyn3", True),
+                                            ("evp2", False), ("maxwell3", False)])
+def test_dynamic_plastic_sensitivity(name, amplifies):
+    """The undamped dynamic plastic models amplify a 1e-14 relative change of
+    the stress to O(1e-1) differences within two steps (elements on the yield
+    surface flip between returning and not), so trajectory comparisons of
+    them beyond one step say nothing about the implementation; the damped
+    quasi-static and the visco-elastic models keep it at rounding level."""
+    m = SC.build(name)
+    a = OracleState(m)
+    b = a.copy()
+    b.f["stress"][:] *= 1 + 1e-14 * (2 * M.splitmix_uniform(5, m.nelem)[:, None] - 1)
+    a.step(2)
+    b.step(2)
+    d = max(SC.rel_err(b.f[k], a.f[k]) for k in ("stress", "vel", "plstrain", "delta_plstrain"))
+    assert (d > 1e-3) if amplifies else (d < 1e-12), d
