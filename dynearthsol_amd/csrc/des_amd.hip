@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <initializer_list>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -2976,6 +2977,84 @@ int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem)
     double best = 1;
     int bi = 0;
     for (int b = 0; b < RED_BLOCKS; ++b)
+        if (hq[b] < best || (hq[b] == best && best < 1 && hi[b] < bi)) {
+            best = hq[b];
+            bi = hi[b];
+        }
+    if (q) *q = best;
+    if (worst_elem) *worst_elem = bi;
+    return DES_OK;
+}
+
+}  // extern "C"
+
+// ---- worst_elem_quality without allocating
+//
+// The reduction's partial results -- one (quality, host element index) pair
+// per block of k_quality_partial's fixed grid of RED_BLOCKS blocks -- in a
+// file-scope device array instead of the context's buffers, for callers that
+// cannot allocate device memory.  Each GPU holds its own copy of the array
+// until the process ends; every call overwrites every slot, so nothing needs
+// clearing, and calls that overlap on one GPU race for it.
+constexpr int NOALLOC_SLOTS = RED_BLOCKS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_SLOTS * (sizeof(double) + sizeof(int));
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[MAX_DEVICES];      // scratch's address on each device
+
+// scratch's device address on the context's device, looked up once per
+// device with that device current (the array's name is only a host-side
+// stand-in for it)
+int scratch_on(des_ctx *ctx, void **ptr)
+{
+    const int dev = ctx->device;
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(ctx, DES_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIPCHK(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(ctx, DES_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *ptr = g_scratch_dev[dev];
+    return DES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int des_worst_quality_noalloc(des_ctx *ctx, double *q, int *worst_elem)
+{
+    if (!ctx) return DES_ERR_INPUT;
+    if (ctx->nmodels)
+        return fail(ctx, DES_ERR_INPUT, "des_worst_quality_noalloc: batched groups use des_batch_worst_quality");
+    void *p = nullptr;
+    int rc = scratch_on(ctx, &p);
+    if (rc) return rc;
+    if ((rc = flush_deferred(ctx))) return rc;      // kernel launches only
+    DevState s = ctx->s;              // the context's state with its partials in scratch
+    s.red = static_cast<double *>(p);
+    s.redi = reinterpret_cast<int *>(static_cast<char *>(p) + NOALLOC_SLOTS * sizeof(double));
+    LAUNCH_ND(k_quality_partial, NOALLOC_SLOTS, s);
+    double hq[NOALLOC_SLOTS];
+    int hi[NOALLOC_SLOTS];
+    HIPCHK(hipMemcpyAsync(hq, s.red, sizeof hq, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hi, s.redi, sizeof hi, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the reference starts from q = 1, element 0, and keeps the first strict minimum
+    double best = 1;
+    int bi = 0;
+    for (int b = 0; b < NOALLOC_SLOTS; ++b)
         if (hq[b] < best || (hq[b] == best && best < 1 && hi[b] < bi)) {
             best = hq[b];
             bi = hi[b];

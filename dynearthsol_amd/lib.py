@@ -29,6 +29,7 @@ EXPORTS = (
     "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
     "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
     "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
+    "des_worst_quality_noalloc",
 )
 
 PER_FUNCTION = (
@@ -72,6 +73,7 @@ def load():
         getattr(lib, "des_" + f).restype = C.c_int
     lib.des_compute_dt.argtypes = [vp, C.POINTER(C.c_double)]
     lib.des_worst_quality.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.des_worst_quality_noalloc.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.des_run.argtypes = [vp, C.c_int, C.c_int]
     lib.des_run_timed.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.des_bad_mesh_quality.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -229,9 +231,12 @@ class DesModel:
         assert counts.shape == (self.E, self.params.nmat)
         self._check(self.lib.des_upload_elemmarkers(self.ctx, abi.iptr(counts)))
 
-    def worst_quality(self):
+    def worst_quality(self, noalloc=False):
+        """worst_elem_quality (geometry.cxx:499-512): (quality, host element);
+        noalloc: through des_worst_quality_noalloc (no device allocation)."""
         q, w = C.c_double(), C.c_int()
-        self._check(self.lib.des_worst_quality(self.ctx, C.byref(q), C.byref(w)))
+        fn = self.lib.des_worst_quality_noalloc if noalloc else self.lib.des_worst_quality
+        self._check(fn(self.ctx, C.byref(q), C.byref(w)))
         return q.value, w.value
 
     def reset(self):

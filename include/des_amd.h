@@ -218,6 +218,14 @@ int des_update_mesh(des_ctx *ctx);          /* dynearthsol.cxx:221-232 */
 int des_rotate_stress(des_ctx *ctx);        /* fields.cxx:411-483 */
 int des_compute_dt(des_ctx *ctx, double *dt_out);   /* geometry.cxx:239-310; sets var.dt */
 int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem); /* geometry.cxx:499-512 */
+/* des_worst_quality without allocating device memory: no counterpart in the
+   reference; for callers that cannot allocate (HIP graph capture, tight
+   serving loops).  The reduction's partial results (1024 quality / element
+   pairs, 12 KiB) live in a static device array that each GPU holds until the
+   process ends and that every call overwrites; calls that overlap on one GPU
+   race for it.  Same results as des_worst_quality; single-model contexts only
+   (a batched group's context is refused with DES_ERR_INPUT). */
+int des_worst_quality_noalloc(des_ctx *ctx, double *q, int *worst_elem);
 /* bad_mesh_quality (remeshing.cxx:1882-1934): *reason = 3 (an element smaller
    than mesh.smallest_size of the resolution), 2 (a bottom node drifted more than
    max_boundary_distortion * resolution, remeshing options 1, 2, 11), 1 (worst
