@@ -79,6 +79,8 @@ def kernel_name(precision, p):
     from dynearthsol_amd import abi
     if precision == abi.DES_MIXED and p in ("pass_c", "pass_e"):
         return f"k_{p}_mixed"
+    if precision == abi.DES_MIXED and p == "pass_f":
+        return "k_pass_f32"
     return "k_dt_partial+k_dt_final" if p == "dt" else "k_" + p
 
 

@@ -574,14 +574,14 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevS
 
 // node part of update_force: gather, stress BCs, elastic foundation,
 // damping (fields.cxx:291-340, bc.cxx:493-566, fields.cxx:235-288)
-template <int ND> __device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND],
-                                                           const double v[ND], bool f32_slots = false)
+template <int ND, bool F32 = false> __device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND],
+                                                                             const double v[ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
     const des_params *P = s.P;
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[d] = 0;
-    if (f32_slots) {
+    if constexpr (F32) {
         // FP32 contributions (mixed precision), accumulated in FP64 in slot order
         const float *fw = reinterpret_cast<const float *>(s.fwork);
         const int p1 = s.csr_off[n + 1];
@@ -1152,7 +1152,6 @@ struct FusedFlags {
     int use_stored_mass = 0;   // first step after a sync: mass/tmass/volume_n from compute_mass
     int store_aux = 0;         // last step before a sync: keep strain_rate, edvoldt, force
     int step_end = 0;          // pass F also ends the step: time, steps, dt_prev, rotation flag
-    int f32_slots = 0;         // force contributions in FP32 slots (mixed precision)
     // the differentiated run's step records, written by the pass that has
     // the value at hand (null in plain runs): pass C (FP64) the stress,
     // plastic strain and volume_old it starts from, pass B the node records
@@ -1487,9 +1486,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, 
     store_force_contrib<ND>(s, e, out);
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, FusedFlags fl)
+template <int ND, bool F32> __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl, int n,
+                                                                     int blk_)
 {
-    BIND_NODE(s, n);
     const double dt = s.sc->dt;
     // one lane per model ends the step; nothing in this pass reads what it updates
     if (fl.step_end && threadIdx.x == 0 &&
@@ -1498,7 +1497,7 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
     double f[ND], v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = __builtin_nontemporal_load(s.vel + d * s.N + n);
-    node_force<ND>(s, n, f, v, fl.f32_slots);
+    node_force<ND, F32>(s, n, f, v);
     if (fl.store_aux) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
@@ -1522,6 +1521,19 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, 
         rn[NrecSlot<ND>::X + d] = xn;
         rn[NrecSlot<ND>::V + d] = v[d];
     }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, FusedFlags fl)
+{
+    BIND_NODE(s, n);
+    pass_f_body<ND, false>(s, fl, n, blk_);
+}
+
+// pass F of the mixed-precision schedule: FP32 force contributions
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f32(DevState s, FusedFlags fl)
+{
+    BIND_NODE(s, n);
+    pass_f_body<ND, true>(s, fl, n, blk_);
 }
 
 // ---- mixed precision (DES_MIXED): passes C and E on (pres, sdev) -------
@@ -3396,8 +3408,8 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     ARM(false);
     FusedFlags ff = fl;
     ff.step_end = 1;
-    ff.f32_slots = mixed;
-    LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, ff);
+    if (mixed) LAUNCH_ND(k_pass_f32, nblocks(ctx->N), ctx->s, ff);
+    else LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, ff);
     MARK(6);
     ctx->steps++;
     ctx->closing_pending = true;
