@@ -95,12 +95,13 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baselines(model, nsteps, runs=3):
+def cpu_baselines(model, nsteps, port_steps, runs=3):
     """The CPU step on this host over a bounded sample of the same workload:
     the reference's own step functions (oracle/_ref, OpenMP build; its
     rheology.cxx is serial because its OpenMP clause does not compile) and
     the oracle's C restatement (every function OpenMP-parallel).  Median of
-    `runs` runs of nsteps steps each, from the same initial state."""
+    `runs` runs of nsteps (reference) and port_steps (restatement) steps each,
+    from the same initial state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
     host = {"cpu": cpu_model(), "nproc": os.cpu_count(), **env}
@@ -129,13 +130,13 @@ def cpu_baselines(model, nsteps, runs=3):
     for _ in range(runs):
         o = base.copy()
         t0 = time.perf_counter()
-        o.step(nsteps)
+        o.step(port_steps)
         secs.append(time.perf_counter() - t0)
     med = statistics.median(secs)
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
-    out.append({"value": model.nelem * nsteps / med, "unit": "elem-steps/s", "cores": threads,
+    out.append({"value": model.nelem * port_steps / med, "unit": "elem-steps/s", "cores": threads,
                 "kind": "port",
-                "sample": f"median of {runs} runs of {nsteps} steps of the same model "
+                "sample": f"median of {runs} runs of {port_steps} steps of the same model "
                           f"({', '.join(f'{s:.2f}' for s in secs)} s): the oracle's C restatement of the "
                           f"step, every function OpenMP-parallel ({threads} threads)",
                 "host": host})
@@ -167,6 +168,7 @@ def bench_ensemble(args):
     print(f"# ensemble: {len(models)} models, {E} elements, built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     bt = DesBatch(models)
+    bt.run(16, abi.DES_MIXED)          # the autotuner (every group's first 8 steps), before the warm-up
     bt.run(args.warmup, abi.DES_MIXED)
     bt.sync()
     t0 = time.perf_counter()
@@ -175,7 +177,9 @@ def bench_ensemble(args):
     secs = time.perf_counter() - t0
     value = E * args.steps / secs
     print(f"# batch: {1e3 * secs / args.steps:.4f} ms per step of all models", file=sys.stderr, flush=True)
-    tuning = {nd: bt.tuning(nd)[0] for nd in (2, 3)}
+    from dynearthsol_amd.lib import TUNE_MEANING
+    tuning = {f"{nd}d": {p: TUNE_MEANING[p][v] for p, (v, _) in bt.tuning_all(nd, abi.DES_MIXED).items()}
+              for nd in (2, 3)}
     bt.close()
     # one at a time: every model, seq_steps timed steps each after a short warmup
     seq_secs, seq_work = 0.0, 0
@@ -300,21 +304,21 @@ def main():
             table["pass_c"] = (be + ELEM_PARAM_BYTES["mixed" if precision == abi.DES_MIXED else "fp64"], bn)
         return table
 
-    # The pass-C autotuner runs before any warm-up, on a throwaway model of
-    # the same mesh; the timed models get its choice.
+    # The autotuner runs before any warm-up, on a throwaway model of the same
+    # mesh (its first 8 steps time every variant of passes A, C, D, E, F);
+    # the timed models get its choices.
+    from dynearthsol_amd.lib import TUNE_MEANING
     tuning = {}
 
     def tuned(precision):
-        if precision == abi.DES_MIXED:
-            return -1           # the mixed pass C has one variant
         if precision not in tuning:
             t = DesModel(m)
             t.run(16, precision)
             t.sync()
-            tuning[precision] = t.tuning()
+            tuning[precision] = t.tuning_all(precision)
             t.close()
-            assert tuning[precision][0] >= 0, tuning
-        return tuning[precision][0]
+            assert all(v >= 0 for v, _ in tuning[precision].values()), tuning
+        return tuning[precision]
 
     # Two identical models per precision.  g's warm-up and the twin's warm-up
     # and timed steps run under per-pass events that the dispatches timestamp
@@ -333,10 +337,10 @@ def main():
             t[1] += times * c
 
     def timed_pair(precision):
-        variant = tuned(precision)
+        choice = tuned(precision)
         a, b = DesModel(m), DesModel(m)
-        a.set_tuning(variant)
-        b.set_tuning(variant)
+        a.set_tuning_all(precision, choice)
+        b.set_tuning_all(precision, choice)
         wa = a.run_timed(args.warmup, precision) if args.warmup else {}
         wb = b.run_timed(args.warmup, precision) if args.warmup else {}
         a.sync()
@@ -395,9 +399,12 @@ def main():
 
     cpu = port = None
     if not args.no_cpu:
-        # about 10 s of CPU work per run at the reference's measured ~3 M elem-steps/s on 16 cores
-        steps = args.cpu_steps or max(1, int(3e7 / E))
-        cpu, port = cpu_baselines(M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem), steps)
+        # about 7 s of CPU work per reference run (measured 3.7 M elem-steps/s on 16 cores)
+        # and about 4 s per run of the restatement (39 M elem-steps/s)
+        steps = args.cpu_steps or max(1, int(2.5e7 / E))
+        port_steps = args.cpu_steps or max(1, int(1.6e8 / E))
+        cpu, port = cpu_baselines(M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem), steps,
+                                  port_steps)
 
     out = {
         "metric": METRIC,
@@ -414,7 +421,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
                    "precision": "fp64" if primary == abi.DES_FP64 else "mixed",
-                   "tuning": {("fp64" if p == abi.DES_FP64 else "mixed"): v for p, v in tuning.items()}},
+                   "tuning": {("fp64" if p == abi.DES_FP64 else "mixed"):
+                              {k: {"variant": TUNE_MEANING[k][v], "ms": [round(x, 5) for x in ms if x > 0]}
+                               for k, (v, ms) in t.items()} for p, t in tuning.items()}},
         "roofline": {"bound": "hbm", "kernel": kernel, "pass": dom, "avg_ms": proc_avg,
                      "launches": tot_n, "timed_window_avg_ms": avg[dom],
                      "bytes_per_launch": be * E + bn * N,

@@ -117,6 +117,25 @@ template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState
     const bool staged = stage_block(s, xcd_block(), lds);  \
     BIND_ELEM(s, e)
 
+// the same with the staging a compile-time choice (STAGE false: every
+// element gathers its node records directly, no LDS)
+#define BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE)                \
+    __shared__ double2 lds[(STAGE) ? STAGE_MAX * (NREC / 2) : 1];   \
+    const bool staged = (STAGE) && stage_block(s, xcd_block(), lds); \
+    BIND_ELEM(s, e)
+
+// Node lane n of a kernel launched in blocks of NB threads (NB divides
+// BLOCK); a batched group's block tables are kept per BLOCK nodes, entry tb_.
+#define BIND_NODE_NB(s, n, NB)                         \
+    const int n = xcd_block() * (NB) + threadIdx.x;    \
+    const int tb_ = n / BLOCK;                         \
+    if (s.nblk_model) {                                \
+        if (n >= s.nblk_lim[tb_]) return;              \
+        bind_model<ND>(s, s.nblk_model[tb_]);          \
+    } else if (n >= s.N) {                             \
+        return;                                        \
+    }
+
 // parameters, scalars and boundary geometry of batched model m
 template <int ND> __device__ __forceinline__ void bind_model(DevState &s, int m)
 {
@@ -1178,9 +1197,9 @@ __device__ __forceinline__ void step_end_scalars(DevScalars *sc, int rheol_type)
 constexpr int REC = 8;
 constexpr int REC_DJ = 4, REC_M = 5, REC_TM = 6, REC_V = 7;
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
 {
-    BIND_ELEM_STAGED(s, e, lds, staged);
+    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const des_params *P = s.P;
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
@@ -1438,9 +1457,9 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
 }
 
 // pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s, FusedFlags fl)
+template <int ND, int NB> __global__ __launch_bounds__(NB) void k_pass_d(DevState s, FusedFlags fl)
 {
-    BIND_NODE(s, n);
+    BIND_NODE_NB(s, n, NB);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
     const int p1 = s.csr_off[n + 1];
@@ -1462,9 +1481,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_d(DevState s, 
     if (fl.rec_dpn) fl.rec_dpn[n] = dpn;
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
 {
-    BIND_ELEM_STAGED(s, e, lds, staged);
+    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
@@ -1487,12 +1506,13 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, 
 }
 
 template <int ND, bool F32> __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl, int n,
-                                                                     int blk_)
+                                                                     int tb_)
 {
     const double dt = s.sc->dt;
-    // one lane per model ends the step; nothing in this pass reads what it updates
-    if (fl.step_end && threadIdx.x == 0 &&
-        (s.nblk_model ? (blk_ == 0 || s.nblk_model[blk_ - 1] != s.nblk_model[blk_]) : blk_ == 0))
+    // the lane of each model's first node ends the step; nothing in this pass
+    // reads what it updates
+    if (fl.step_end && n % BLOCK == 0 &&
+        (s.nblk_model ? (tb_ == 0 || s.nblk_model[tb_ - 1] != s.nblk_model[tb_]) : n == 0))
         step_end_scalars(s.sc, s.P->rheol_type);
     double f[ND], v[ND];
 #pragma unroll
@@ -1523,17 +1543,17 @@ template <int ND, bool F32> __device__ __forceinline__ void pass_f_body(DevState
     }
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f(DevState s, FusedFlags fl)
+template <int ND, int NB> __global__ __launch_bounds__(NB) void k_pass_f(DevState s, FusedFlags fl)
 {
-    BIND_NODE(s, n);
-    pass_f_body<ND, false>(s, fl, n, blk_);
+    BIND_NODE_NB(s, n, NB);
+    pass_f_body<ND, false>(s, fl, n, tb_);
 }
 
 // pass F of the mixed-precision schedule: FP32 force contributions
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_f32(DevState s, FusedFlags fl)
+template <int ND, int NB> __global__ __launch_bounds__(NB) void k_pass_f32(DevState s, FusedFlags fl)
 {
-    BIND_NODE(s, n);
-    pass_f_body<ND, true>(s, fl, n, blk_);
+    BIND_NODE_NB(s, n, NB);
+    pass_f_body<ND, true>(s, fl, n, tb_);
 }
 
 // ---- mixed precision (DES_MIXED): passes C and E on (pres, sdev) -------
@@ -1712,9 +1732,9 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
     }
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
 {
-    BIND_ELEM_STAGED(s, e, lds, staged);
+    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
@@ -1807,9 +1827,12 @@ struct des_ctx {
     size_t xfer_cap = 0;           // its size in doubles
     std::vector<int> h_conn;       // host copies for output frames
     std::vector<unsigned> h_bcflag;
-    int passc_variant = -1;        // waves-per-EU variant of pass C chosen by the autotuner
-    double passc_ms[4] = {0, 0, 0, 0};
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> tune;  // autotuning launches in flight
+    // the autotuner's choice of variant per tunable fused pass (DES_TUNE_*
+    // order: A, C, D, E, F), per precision (0 FP64, 1 mixed); -1 untuned
+    int tune_var[2][DES_NTUNE] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
+    double tune_ms[2][DES_NTUNE][4] = {};   // mean ms per variant it measured
+    struct TuneLaunch { int pass, variant; hipEvent_t a, b; };
+    std::vector<TuneLaunch> tune;          // timed tuning launches in flight
     // batched group (nmodels > 0): models of one dimension concatenated, each
     // starting on a block boundary of the element and of the node arrays
     int nmodels = 0;
@@ -2821,6 +2844,62 @@ int write_frame_impl(des_ctx *ctx, const char *modelname, int frame, double run_
     return DES_OK;
 }
 
+// Output::write_checkpoint (output.cxx:254-276) in the format of binaryio.cxx
+int write_checkpoint_impl(des_ctx *ctx, const char *modelname, int frame, int nseg, const int *segment,
+                          const int *segflag, int nextra, const des_named_array *extra)
+{
+    if (!ctx || !modelname || frame < 0 || nseg < 0 || (nseg && (!segment || !segflag)) || nextra < 0 ||
+        (nextra && !extra) || ctx->nmodels)
+        return DES_ERR_INPUT;
+    int rc = flush_deferred(ctx);
+    if (rc) return rc;
+    const long E = ctx->E;
+    std::vector<double> vold(E), syy;
+    des_fields f{};
+    f.volume_old = vold.data();
+    if (ctx->P.is_plane_strain) {
+        syy.resize(E);
+        f.stressyy = syy.data();
+    }
+    if ((rc = des_download_fields(ctx, &f))) return rc;
+    DevScalars sc{};
+    if ((rc = read_scalars(ctx, sc))) return rc;
+    const double tcp[2] = {sc.time, sc.compensation_pressure};
+
+    char fname[1024];
+    std::snprintf(fname, sizeof fname, "%s.chkpt.%06d", modelname, frame);
+    std::FILE *fp = std::fopen(fname, "wb");
+    if (!fp) return fail(ctx, DES_ERR_IO, std::string("cannot open ") + fname);
+    const size_t headerlen = 4096;
+    std::string header = "# DynEarthSol ndims=" + std::to_string(ctx->nd) + " revision=3\n";
+    long eof = (long)headerlen;
+    std::fseek(fp, eof, SEEK_SET);
+    bool ok = true;
+    auto put = [&](const char *name, const void *data, size_t bytes) {
+        char line[256];
+        std::snprintf(line, sizeof line, "%s\t%ld\n", name, eof);
+        header += line;
+        if (bytes && std::fwrite(data, 1, bytes, fp) != bytes) ok = false;
+        eof += (long)bytes;
+    };
+    put("time compensation_pressure", tcp, sizeof tcp);
+    put("segment", segment, sizeof(int) * (size_t)nseg * ctx->nd);
+    put("segflag", segflag, sizeof(int) * (size_t)nseg);
+    put("volume_old", vold.data(), sizeof(double) * vold.size());
+    if (ctx->P.is_plane_strain) put("stressyy", syy.data(), sizeof(double) * syy.size());
+    for (int k = 0; k < nextra; ++k) {
+        if (!extra[k].name || (extra[k].bytes && !extra[k].data)) ok = false;
+        else put(extra[k].name, extra[k].data, extra[k].bytes);
+    }
+    if (header.size() >= headerlen) ok = false;
+    header.resize(headerlen, '\0');
+    std::fseek(fp, 0, SEEK_SET);
+    if (std::fwrite(header.data(), 1, headerlen, fp) != headerlen) ok = false;
+    std::fclose(fp);
+    if (!ok) return fail(ctx, DES_ERR_IO, std::string("failed writing ") + fname);
+    return DES_OK;
+}
+
 // Output::average_fields for the step just completed (ctx->steps is var.steps)
 int average_fields(des_ctx *ctx)
 {
@@ -2839,6 +2918,12 @@ extern "C" {
 int des_write_frame(des_ctx *ctx, const char *modelname, int frame, double run_time)
 {
     return write_frame_impl(ctx, modelname, frame, run_time, false);
+}
+
+int des_write_checkpoint(des_ctx *ctx, const char *modelname, int frame, int nseg, const int *segment,
+                         const int *segflag, int nextra, const des_named_array *extra)
+{
+    return write_checkpoint_impl(ctx, modelname, frame, nseg, segment, segflag, nextra, extra);
 }
 
 int des_write_frame_averaged(des_ctx *ctx, const char *modelname, int frame, double run_time)
@@ -3259,10 +3344,94 @@ int ensure_mixed(des_ctx *ctx)
     return DES_OK;
 }
 
-// Pass C with a minimum of 1, 2, 3 or 4 waves per SIMD: fewer registers per
-// lane buy occupancy at the price of spills.  All variants perform the same
-// arithmetic; the autotuner below keeps the fastest.
+// ---- the autotuner's variants of the fused passes.  All variants of a pass
+// perform the same arithmetic in the same order (bitwise equal results);
+// they differ in how the work maps on the machine:
+//   A, E: the block's node records staged in LDS (0) or gathered directly (1);
+//   C:    a minimum of 1, 2, 3 or 4 waves per SIMD (__launch_bounds__): fewer
+//         registers per lane buy occupancy at the price of spills (FP64 only;
+//         the mixed pass C has one variant);
+//   D, F: node blocks of 256, 128 or 64 threads.
+// The first TUNE_STEPS steps of a run of at least TUNE_MIN_RUN steps time
+// every variant of every untuned pass with HIP events (variant k mod the
+// variant count at tuning step k) and keep the fastest mean.
+constexpr int TUNE_A = DES_TUNE_A, TUNE_C = DES_TUNE_C, TUNE_D = DES_TUNE_D, TUNE_E = DES_TUNE_E,
+              TUNE_F = DES_TUNE_F;
+constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 4, 3, 2, 3}, {2, 1, 3, 2, 3}};
+constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 0, 0, 0};
+constexpr int TUNE_STEPS = 8, TUNE_MIN_RUN = 16;
 constexpr int PASSC_VARIANTS = 4;
+
+void clear_tuning_events(des_ctx *ctx)
+{
+    for (auto &t : ctx->tune) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    ctx->tune.clear();
+}
+
+int tuned_variant(const des_ctx *ctx, bool mixed, int pass)
+{
+    const int v = ctx->tune_var[mixed ? 1 : 0][pass];
+    return v < 0 ? std::min(TUNE_DEFAULT[pass], TUNE_NV[mixed ? 1 : 0][pass] - 1) : v;
+}
+
+// kernel<ND, P2> in blocks of nthreads
+#define LAUNCH_T2(kern, P2, nthreads, grid, ...)                                                    \
+    do {                                                                                          \
+        if (ctx->nd == 3)                                                                         \
+            hipExtLaunchKernelGGL((kern<3, P2>), dim3(grid), dim3(nthreads), 0, ctx->stream,      \
+                                  ctx->ev_a, ctx->ev_b, 0, __VA_ARGS__);                          \
+        else                                                                                      \
+            hipExtLaunchKernelGGL((kern<2, P2>), dim3(grid), dim3(nthreads), 0, ctx->stream,      \
+                                  ctx->ev_a, ctx->ev_b, 0, __VA_ARGS__);                          \
+        int rc_ = check_launch(ctx);                                                              \
+        if (rc_) return rc_;                                                                      \
+    } while (0)
+
+int launch_pass_a(des_ctx *ctx, const FusedFlags &fl, int variant)
+{
+    if (variant == 1) LAUNCH_T2(k_pass_a, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    else LAUNCH_T2(k_pass_a, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    return DES_OK;
+}
+
+int launch_pass_d(des_ctx *ctx, const FusedFlags &fl, int variant)
+{
+    const long N = ctx->N;
+    if (variant == 2) LAUNCH_T2(k_pass_d, 64, 64, (N + 63) / 64, ctx->s, fl);
+    else if (variant == 1) LAUNCH_T2(k_pass_d, 128, 128, (N + 127) / 128, ctx->s, fl);
+    else LAUNCH_T2(k_pass_d, 256, 256, (N + 255) / 256, ctx->s, fl);
+    return DES_OK;
+}
+
+int launch_pass_e(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
+{
+    if (mixed) {
+        if (variant == 1) LAUNCH_T2(k_pass_e_mixed, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
+        else LAUNCH_T2(k_pass_e_mixed, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    } else {
+        if (variant == 1) LAUNCH_T2(k_pass_e, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
+        else LAUNCH_T2(k_pass_e, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    }
+    return DES_OK;
+}
+
+int launch_pass_f(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
+{
+    const long N = ctx->N;
+    if (mixed) {
+        if (variant == 2) LAUNCH_T2(k_pass_f32, 64, 64, (N + 63) / 64, ctx->s, fl);
+        else if (variant == 1) LAUNCH_T2(k_pass_f32, 128, 128, (N + 127) / 128, ctx->s, fl);
+        else LAUNCH_T2(k_pass_f32, 256, 256, (N + 255) / 256, ctx->s, fl);
+    } else {
+        if (variant == 2) LAUNCH_T2(k_pass_f, 64, 64, (N + 63) / 64, ctx->s, fl);
+        else if (variant == 1) LAUNCH_T2(k_pass_f, 128, 128, (N + 127) / 128, ctx->s, fl);
+        else LAUNCH_T2(k_pass_f, 256, 256, (N + 255) / 256, ctx->s, fl);
+    }
+    return DES_OK;
+}
 
 int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
 {
@@ -3329,6 +3498,7 @@ int fused_begin(des_ctx *ctx, int precision)
     int rc = flush_deferred(ctx);
     if (rc) return rc;
     if ((rc = ensure_retlist(ctx))) return rc;
+    clear_tuning_events(ctx);          // left by a run that stopped while tuning
     LAUNCH_ND(k_pack_nrec, nblocks(ctx->N), ctx->s);
     if (precision == DES_MIXED) {
         if ((rc = ensure_mixed(ctx))) return rc;
@@ -3346,47 +3516,44 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     fl.use_stored_mass = (k == 0);
     fl.store_aux = (k == nsteps - 1);
     if (k > 0) swap_volume(ctx);
+    // a tuning step: the untuned passes run variant k mod their variant count
+    // between HIP events
+    const int pr = mixed ? 1 : 0;
+    bool tuning = false;
+    if (k < TUNE_STEPS && nsteps >= TUNE_MIN_RUN)
+        for (int p = 0; p < DES_NTUNE; ++p) tuning |= ctx->tune_var[pr][p] < 0;
+    auto tunable = [&](int p, auto &&launch) -> int {
+        const bool timed = tuning && ctx->tune_var[pr][p] < 0;
+        const int v = timed ? k % TUNE_NV[pr][p] : tuned_variant(ctx, mixed, p);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timed) {
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            HIPCHK(hipEventRecord(a, ctx->stream));
+        }
+        const int rc2 = launch(v);
+        if (rc2) return rc2;
+        if (timed) {
+            HIPCHK(hipEventRecord(b, ctx->stream));
+            ctx->tune.push_back({p, v, a, b});
+        }
+        return DES_OK;
+    };
     ARM(false);
-    LAUNCH_ND(k_pass_a, nblocks(ctx->E), ctx->s, fl);
+    if ((rc = tunable(TUNE_A, [&](int v) { return launch_pass_a(ctx, fl, v); }))) return rc;
     MARK(0);
     ARM(false);
     LAUNCH_ND(k_pass_b, nblocks(4L * ctx->N), ctx->s, fl);
     MARK(1);
-    auto &tune = ctx->tune;
     ARM(false);
-    if (mixed) {
-        LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
-    } else if (ctx->passc_variant < 0 && tune.size() < 2 * PASSC_VARIANTS && nsteps >= 16) {
-        // autotuning: the first 2 x 4 steps of a long enough run time each variant twice
-        const int var = (int)tune.size() % PASSC_VARIANTS;
-        hipEvent_t a, b;
-        HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
-        HIPCHK(hipEventRecord(a, ctx->stream));
-        if ((rc = launch_pass_c(ctx, fl, var))) return rc;
-        HIPCHK(hipEventRecord(b, ctx->stream));
-        tune.push_back({a, b});
-        if (tune.size() == 2 * PASSC_VARIANTS) {
-            HIPCHK(hipEventSynchronize(b));
-            double best = 1e300;
-            for (int v = 0; v < PASSC_VARIANTS; ++v) ctx->passc_ms[v] = 0;
-            for (size_t j = 0; j < tune.size(); ++j) {
-                float t = 0;
-                HIPCHK(hipEventElapsedTime(&t, tune[j].first, tune[j].second));
-                ctx->passc_ms[j % PASSC_VARIANTS] += t;
-                (void)hipEventDestroy(tune[j].first);
-                (void)hipEventDestroy(tune[j].second);
-            }
-            tune.clear();
-            for (int v = 0; v < PASSC_VARIANTS; ++v)
-                if (ctx->passc_ms[v] < best) {
-                    best = ctx->passc_ms[v];
-                    ctx->passc_variant = v;
-                }
+    rc = tunable(TUNE_C, [&](int v) -> int {
+        if (mixed) {
+            LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
+            return DES_OK;
         }
-    } else {
-        if ((rc = launch_pass_c(ctx, fl, ctx->passc_variant < 0 ? 1 : ctx->passc_variant))) return rc;
-    }
+        return launch_pass_c(ctx, fl, v);
+    });
+    if (rc) return rc;
     MARK(2);
     if (ctx->nd == 3 && any_plastic(ctx)) {
         ARM(false);
@@ -3394,11 +3561,10 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         MARK(8);
     }
     ARM(false);
-    LAUNCH_ND(k_pass_d, nblocks(ctx->N), ctx->s, fl);
+    if ((rc = tunable(TUNE_D, [&](int v) { return launch_pass_d(ctx, fl, v); }))) return rc;
     MARK(3);
     ARM(false);
-    if (mixed) LAUNCH_ND(k_pass_e_mixed, nblocks(ctx->E), ctx->s, fl);
-    else LAUNCH_ND(k_pass_e, nblocks(ctx->E), ctx->s, fl);
+    if ((rc = tunable(TUNE_E, [&](int v) { return launch_pass_e(ctx, fl, mixed, v); }))) return rc;
     MARK(4);
     if (ctx->nactive) {
         ARM(false);
@@ -3408,9 +3574,30 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     ARM(false);
     FusedFlags ff = fl;
     ff.step_end = 1;
-    if (mixed) LAUNCH_ND(k_pass_f32, nblocks(ctx->N), ctx->s, ff);
-    else LAUNCH_ND(k_pass_f, nblocks(ctx->N), ctx->s, ff);
+    if ((rc = tunable(TUNE_F, [&](int v) { return launch_pass_f(ctx, ff, mixed, v); }))) return rc;
     MARK(6);
+    if (tuning && k == TUNE_STEPS - 1 && !ctx->tune.empty()) {
+        // the fastest mean per untuned pass
+        HIPCHK(hipEventSynchronize(ctx->tune.back().b));
+        double sum[DES_NTUNE][4] = {};
+        int cnt[DES_NTUNE][4] = {};
+        for (auto &t : ctx->tune) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+            sum[t.pass][t.variant] += ms;
+            cnt[t.pass][t.variant]++;
+        }
+        clear_tuning_events(ctx);
+        for (int p = 0; p < DES_NTUNE; ++p) {
+            if (ctx->tune_var[pr][p] >= 0) continue;
+            int best = -1;
+            for (int v = 0; v < TUNE_NV[pr][p]; ++v) {
+                ctx->tune_ms[pr][p][v] = cnt[p][v] ? sum[p][v] / cnt[p][v] : 0;
+                if (cnt[p][v] && (best < 0 || ctx->tune_ms[pr][p][v] < ctx->tune_ms[pr][p][best])) best = v;
+            }
+            ctx->tune_var[pr][p] = best < 0 ? TUNE_DEFAULT[p] : best;
+        }
+    }
     ctx->steps++;
     ctx->closing_pending = true;
     if (ctx->steps % 10 == 0) {
@@ -3453,7 +3640,30 @@ int des_run(des_ctx *ctx, int nsteps, int precision)
 int des_set_tuning(des_ctx *ctx, int variant)
 {
     if (!ctx || variant < -1 || variant >= PASSC_VARIANTS) return DES_ERR_INPUT;
-    ctx->passc_variant = variant;
+    ctx->tune_var[0][TUNE_C] = variant;
+    return DES_OK;
+}
+
+int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE], double ms[4 * DES_NTUNE])
+{
+    if (!ctx || !variant || (precision != DES_FP64 && precision != DES_MIXED)) return DES_ERR_INPUT;
+    const int pr = precision == DES_MIXED ? 1 : 0;
+    for (int p = 0; p < DES_NTUNE; ++p) {
+        variant[p] = ctx->tune_var[pr][p];
+        if (ms)
+            for (int v = 0; v < 4; ++v) ms[4 * p + v] = ctx->tune_ms[pr][p][v];
+    }
+    return DES_OK;
+}
+
+int des_set_tuning_all(des_ctx *ctx, int precision, const int variant[DES_NTUNE])
+{
+    if (!ctx || !variant || (precision != DES_FP64 && precision != DES_MIXED)) return DES_ERR_INPUT;
+    const int pr = precision == DES_MIXED ? 1 : 0;
+    for (int p = 0; p < DES_NTUNE; ++p)
+        if (variant[p] < -1 || variant[p] >= TUNE_NV[pr][p])
+            return fail(ctx, DES_ERR_INPUT, "tuning variant out of range for pass " + std::to_string(p));
+    for (int p = 0; p < DES_NTUNE; ++p) ctx->tune_var[pr][p] = variant[p];
     return DES_OK;
 }
 
@@ -3830,21 +4040,24 @@ int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4])
 {
     if (!b || !variant) return DES_ERR_INPUT;
     for (des_ctx *ctx : b->groups)
-        if (ctx->nd == ndims) {
-            *variant = ctx->passc_variant;
-            if (ms)
-                for (int v = 0; v < 4; ++v) ms[v] = ctx->passc_ms[v];
-            return DES_OK;
-        }
+        if (ctx->nd == ndims) return des_get_tuning(ctx, variant, ms);
+    return bfail(b, DES_ERR_INPUT, "no group of that dimension");
+}
+
+int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES_NTUNE], double ms[4 * DES_NTUNE])
+{
+    if (!b || !variant) return DES_ERR_INPUT;
+    for (des_ctx *ctx : b->groups)
+        if (ctx->nd == ndims) return des_get_tuning_all(ctx, precision, variant, ms);
     return bfail(b, DES_ERR_INPUT, "no group of that dimension");
 }
 
 int des_get_tuning(des_ctx *ctx, int *variant, double ms[4])
 {
     if (!ctx || !variant) return DES_ERR_INPUT;
-    *variant = ctx->passc_variant;
+    *variant = ctx->tune_var[0][TUNE_C];
     if (ms)
-        for (int v = 0; v < 4; ++v) ms[v] = ctx->passc_ms[v];
+        for (int v = 0; v < 4; ++v) ms[v] = ctx->tune_ms[0][TUNE_C][v];
     return DES_OK;
 }
 

@@ -29,8 +29,16 @@ EXPORTS = (
     "des_batch_compute_dt", "des_batch_run", "des_batch_sync", "des_batch_worst_quality",
     "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
     "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
-    "des_worst_quality_noalloc",
+    "des_worst_quality_noalloc", "des_get_tuning_all", "des_set_tuning_all", "des_batch_tuning_all",
 )
+
+NTUNE = 5
+TUNE_PASSES = ("pass_a", "pass_c", "pass_d", "pass_e", "pass_f")
+TUNE_MEANING = {
+    "pass_a": ("lds-staged", "direct"), "pass_e": ("lds-staged", "direct"),
+    "pass_c": ("waves>=1", "waves>=2", "waves>=3", "waves>=4"),
+    "pass_d": ("block256", "block128", "block64"), "pass_f": ("block256", "block128", "block64"),
+}
 
 PER_FUNCTION = (
     "compute_volume", "compute_mass", "compute_shape_fn", "init_derived",
@@ -84,6 +92,9 @@ def load():
     lib.des_average_fields.argtypes = [vp]
     lib.des_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.des_set_tuning.argtypes = [vp, C.c_int]
+    lib.des_get_tuning_all.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    lib.des_set_tuning_all.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    lib.des_batch_tuning_all.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.des_set_mesh_order.argtypes = [C.c_int]
     lib.des_set_mesh_order.restype = None
     lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
@@ -271,6 +282,20 @@ class DesModel:
     def adj_release(self):
         self.lib.des_adj_release(self.ctx)
 
+    def tuning_all(self, precision=abi.DES_FP64):
+        """{pass: (variant or -1, [mean ms per variant])} of the autotuner."""
+        v = (C.c_int * NTUNE)()
+        ms = (C.c_double * (4 * NTUNE))()
+        self._check(self.lib.des_get_tuning_all(self.ctx, precision, v, ms))
+        return {p: (v[k], list(ms[4 * k:4 * k + 4])) for k, p in enumerate(TUNE_PASSES)}
+
+    def set_tuning_all(self, precision, choices):
+        """Fix the autotuner's choices ({pass: variant} or a sequence in
+        TUNE_PASSES order; -1 lets the next long run tune that pass)."""
+        if isinstance(choices, dict):
+            choices = [choices[p][0] if isinstance(choices[p], tuple) else choices[p] for p in TUNE_PASSES]
+        self._check(self.lib.des_set_tuning_all(self.ctx, precision, (C.c_int * NTUNE)(*choices)))
+
     def tuning(self):
         """(pass-C variant chosen by the autotuner or -1, [ms per variant])."""
         v = C.c_int()
@@ -405,6 +430,13 @@ class DesBatch:
         ms = (C.c_double * 4)()
         self._check(self.lib.des_batch_tuning(self.b, ndims, C.byref(v), ms))
         return v.value, list(ms)
+
+    def tuning_all(self, ndims, precision=abi.DES_FP64):
+        """{pass: (variant or -1, [mean ms per variant])} of the group of that dimension."""
+        v = (C.c_int * NTUNE)()
+        ms = (C.c_double * (4 * NTUNE))()
+        self._check(self.lib.des_batch_tuning_all(self.b, ndims, precision, v, ms))
+        return {p: (v[k], list(ms[4 * k:4 * k + 4])) for k, p in enumerate(TUNE_PASSES)}
 
     def download(self, k, names=None):
         out, f = _field_buffers(self.models[k], names)

@@ -17,6 +17,8 @@
 #ifndef DES_AMD_H
 #define DES_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -186,6 +188,25 @@ int des_set_averaging(des_ctx *ctx, int average_interval);
 int des_average_fields(des_ctx *ctx);
 int des_write_frame_averaged(des_ctx *ctx, const char *modelname, int frame, double run_time);
 
+/* A named block of bytes written into a checkpoint as it is. */
+typedef struct {
+    const char *name;
+    const void *data;
+    size_t bytes;
+} des_named_array;
+
+/* Output::write_checkpoint (output.cxx:254-276): writes
+   "<modelname>.chkpt.<frame %06d>" in the binary format with "time
+   compensation_pressure", the host's boundary segments ("segment", nseg x
+   ndims, and "segflag", nseg) as given, "volume_old" from the device and, in
+   plane strain, "stressyy" -- then the nextra host-owned arrays in order (the
+   marker sets' "<name> size", ".eta", ".elem", ".mattype", ".id" of
+   MarkerSet::write_chkpt_file, markerset.cxx:386-398, which the host keeps).
+   With the frame of the same number from des_write_frame this is what
+   restart() (dynearthsol.cxx:104-218) reads. */
+int des_write_checkpoint(des_ctx *ctx, const char *modelname, int frame, int nseg, const int *segment,
+                         const int *segflag, int nextra, const des_named_array *extra);
+
 /* Marker counts per element and material (var.elemmarkers, nelem x nmat),
    e.g. after the host's phase changes (phasechanges.cxx:232-258). */
 int des_upload_elemmarkers(des_ctx *ctx, const int *counts);
@@ -257,11 +278,32 @@ int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[10], int co
    next compute_dt, every 10 steps). */
 int des_sync(des_ctx *ctx);
 
-/* The pass-C variant (minimum waves per SIMD 1..4 as 0..3) that the autotuner
-   of des_run chose, -1 before a run of 16 or more steps, and the milliseconds
-   it measured for each.  No counterpart in the reference. */
+/* The autotuner of des_run (no counterpart in the reference).  The tunable
+   fused passes and their variants -- every variant performs the same
+   arithmetic in the same order, so results do not depend on the choice:
+     DES_TUNE_A, DES_TUNE_E: 0 the element block's node records staged in LDS,
+                             1 gathered directly;
+     DES_TUNE_C:             FP64 pass C with a minimum of 1..4 waves per SIMD
+                             (0..3); the mixed pass C has one variant (0);
+     DES_TUNE_D, DES_TUNE_F: node blocks of 256, 128 or 64 threads (0..2).
+   The first 8 steps of a run of 16 or more steps time every variant of each
+   pass not tuned yet (variant k mod the variant count at step k) and keep the
+   fastest mean; FP64 and mixed-precision runs tune separately, and each
+   dimension's group of a batch tunes on its own. */
+#define DES_NTUNE 5
+#define DES_TUNE_A 0
+#define DES_TUNE_C 1
+#define DES_TUNE_D 2
+#define DES_TUNE_E 3
+#define DES_TUNE_F 4
+/* variant[p]: the choice for pass p at this precision (DES_FP64 or DES_MIXED),
+   -1 while untuned; ms[4 * p + v]: the mean milliseconds measured for
+   variant v of pass p (0 where not measured). */
+int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE], double ms[4 * DES_NTUNE]);
+/* Fix the choices (each -1 to let the next long run tune that pass). */
+int des_set_tuning_all(des_ctx *ctx, int precision, const int variant[DES_NTUNE]);
+/* The FP64 pass-C entry of the above: variant 0..3 or -1, ms per variant. */
 int des_get_tuning(des_ctx *ctx, int *variant, double ms[4]);
-/* Fix the pass-C variant (0..3), or -1 to let the next long run autotune. */
 int des_set_tuning(des_ctx *ctx, int variant);
 
 /* ---- differentiated run (no counterpart in the reference) ------------------
@@ -324,6 +366,8 @@ int des_batch_run(des_batch *b, int nsteps, int precision);
 int des_batch_sync(des_batch *b);
 int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem);   /* per model */
 int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4]);
+int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES_NTUNE],
+                         double ms[4 * DES_NTUNE]);
 
 #ifdef __cplusplus
 }

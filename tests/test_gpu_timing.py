@@ -58,3 +58,67 @@ def test_autotuner_keeps_the_fastest_variant():
     assert 0 <= v < 4 and all(x > 0 for x in ms) and ms[v] == min(ms)
     with pytest.raises(DesError):
         g.set_tuning(4)
+
+
+VARIANTS = {"pass_a": 2, "pass_c": 4, "pass_d": 3, "pass_e": 2, "pass_f": 3}
+
+
+@pytest.mark.parametrize("name,prec", [("evp3", abi.DES_FP64), ("evp3", abi.DES_MIXED), ("mat2", abi.DES_FP64),
+                                       ("bc3", abi.DES_FP64)])
+def test_every_variant_of_every_pass_gives_identical_results(name, prec):
+    """Each tunable pass run in each of its variants (the others at their
+    defaults) leaves bitwise the same state after 20 steps."""
+    from dynearthsol_amd.lib import DesModel, TUNE_PASSES
+    m = SC.build(name)
+    ref = DesModel(m)
+    ref.set_tuning_all(prec, [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0])
+    ref.run(20, prec)
+    a = ref.download()
+    for p in TUNE_PASSES:
+        nv = 1 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
+        for v in range(nv):
+            choice = [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0]
+            choice[TUNE_PASSES.index(p)] = v
+            g = DesModel(m)
+            g.set_tuning_all(prec, choice)
+            g.run(20, prec)
+            b = g.download()
+            for k in a:
+                assert np.array_equal(a[k], b[k]), (p, v, k)
+            assert g.scalars() == ref.scalars()
+            g.close()
+
+
+@pytest.mark.parametrize("prec", [abi.DES_FP64, abi.DES_MIXED])
+def test_autotuner_tunes_every_pass(prec):
+    """A run of 16+ steps tunes every pass: the choice is the variant with
+    the smallest measured mean, and the tuned run equals an untuned one."""
+    from dynearthsol_amd.lib import DesModel
+    m = SC.build("evp3")
+    g, h = DesModel(m), DesModel(m)
+    assert all(v == -1 for v, _ in g.tuning_all(prec).values())
+    g.run(20, prec)
+    h.set_tuning_all(prec, [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0])
+    h.run(20, prec)
+    t = g.tuning_all(prec)
+    print(prec, {p: (v, [round(x, 4) for x in ms]) for p, (v, ms) in t.items()}, flush=True)
+    for p, (v, ms) in t.items():
+        nv = 1 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
+        assert 0 <= v < nv, (p, v)
+        assert all(x > 0 for x in ms[:nv]) and ms[v] == min(ms[:nv]), (p, ms)
+    a, b = g.download(), h.download()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    with pytest.raises(Exception):
+        g.set_tuning_all(prec, [2, 0, 0, 0, 0])
+
+
+def test_batched_groups_tune_every_pass():
+    from dynearthsol_amd.lib import DesBatch
+    bt = DesBatch([SC.build(n) for n in ("evp3", "maxwell3", "evp2", "viscous2")])
+    bt.run(20, abi.DES_MIXED)
+    bt.sync()
+    for nd in (2, 3):
+        t = bt.tuning_all(nd, abi.DES_MIXED)
+        print(nd, {p: v for p, (v, _) in t.items()}, flush=True)
+        assert all(v >= 0 for v, _ in t.values()), t
