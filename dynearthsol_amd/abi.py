@@ -127,3 +127,8 @@ def uptr(a):
         return None
     assert a.dtype.name == "uint32" and a.flags.c_contiguous
     return a.ctypes.data_as(_P(C.c_uint))
+
+
+class DesNamedArray(C.Structure):
+    """des_named_array: a named block of bytes for des_write_checkpoint."""
+    _fields_ = [("name", C.c_char_p), ("data", C.c_void_p), ("bytes", C.c_size_t)]

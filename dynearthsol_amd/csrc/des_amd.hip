@@ -995,6 +995,13 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final_b(DevState
     }
 }
 
+// the bottom nodes' coord0 after a remesh (remeshing.cxx:2020-2026)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_reset_bottom_coord0(DevState s)
+{
+    BIND_NODE(s, n);
+    if (s.bcflag[n] & BOUNDZ0) s.coord0[(ND - 1) * s.N + n] = -s.P->zlength;
+}
+
 // ---- worst_elem_quality (geometry.cxx:458-512)
 template <int ND> __device__ __forceinline__ double elem_quality(const DevState &s, int e)
 {
@@ -3043,6 +3050,16 @@ int des_init_derived(des_ctx *ctx)
     HIPCHK(hipMemcpyAsync(ctx->s.volume_old, ctx->s.volume, sizeof(double) * ctx->E, hipMemcpyDeviceToDevice, ctx->stream));
     if ((rc = launch_mass(ctx))) return rc;
     LAUNCH_ND(k_shape_fn, nblocks(ctx->E), ctx->s);
+    return DES_OK;
+}
+
+int des_remesh_finish(des_ctx *ctx, int remeshing_option)
+{
+    int rc = des_init_derived(ctx);
+    if (rc) return rc;
+    if (remeshing_option == 1 || remeshing_option == 2 || remeshing_option == 11)
+        LAUNCH_ND(k_reset_bottom_coord0, nblocks(ctx->N), ctx->s);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
 }
 

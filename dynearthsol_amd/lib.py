@@ -30,6 +30,7 @@ EXPORTS = (
     "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
     "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
     "des_worst_quality_noalloc", "des_get_tuning_all", "des_set_tuning_all", "des_batch_tuning_all",
+    "des_write_checkpoint", "des_remesh_finish",
 )
 
 NTUNE = 5
@@ -88,6 +89,9 @@ def load():
     lib.des_upload_elemmarkers.argtypes = [vp, C.POINTER(C.c_int)]
     lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     lib.des_write_frame_averaged.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
+    lib.des_remesh_finish.argtypes = [vp, C.c_int]
+    lib.des_write_checkpoint.argtypes = [vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.c_int, C.POINTER(abi.DesNamedArray)]
     lib.des_set_averaging.argtypes = [vp, C.c_int]
     lib.des_average_fields.argtypes = [vp]
     lib.des_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
@@ -224,6 +228,27 @@ class DesModel:
     def write_frame(self, modelname, frame, run_time=0.0):
         """<modelname>.save.<frame> and a line of <modelname>.info (output.cxx:79-208)."""
         self._check(self.lib.des_write_frame(self.ctx, str(modelname).encode(), frame, run_time))
+
+    def remesh_finish(self, remeshing_option=0):
+        """remesh()'s tail on the device (des_remesh_finish)."""
+        self._check(self.lib.des_remesh_finish(self.ctx, remeshing_option))
+
+    def write_checkpoint(self, modelname, frame, segment, segflag, extra=()):
+        """<modelname>.chkpt.<frame> (output.cxx:254-276): the host's boundary
+        segments (nseg x ndims, int) and flags (nseg), then the named byte
+        blocks of extra ((name, bytes) pairs: the host's marker sets)."""
+        seg = np.ascontiguousarray(segment, dtype=np.int32).reshape(-1, self.nd)
+        flg = np.ascontiguousarray(segflag, dtype=np.int32).reshape(-1)
+        assert len(seg) == len(flg)
+        blocks = [(str(n).encode(), bytes(b)) for n, b in extra]
+        arr = (abi.DesNamedArray * max(len(blocks), 1))()
+        keep = []
+        for k, (n, b) in enumerate(blocks):
+            buf = C.create_string_buffer(b, len(b))
+            keep.append(buf)
+            arr[k].name, arr[k].data, arr[k].bytes = n, C.cast(buf, C.c_void_p), len(b)
+        self._check(self.lib.des_write_checkpoint(self.ctx, str(modelname).encode(), frame, len(flg),
+                                                  abi.iptr(seg), abi.iptr(flg), len(blocks), arr))
 
     def write_frame_averaged(self, modelname, frame, run_time=0.0):
         """A regular frame with the averaged fields (output.cxx:79-208)."""

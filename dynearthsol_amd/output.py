@@ -64,3 +64,33 @@ def read_info(path):
             rows.append((int(v[0]), int(v[1]), float(v[2]), float(v[3]), float(v[4]),
                          int(v[5]), int(v[6]), int(v[7])))
     return rows
+
+
+CHKPT_DTYPES = {"time compensation_pressure": np.float64, "segment": np.int32, "segflag": np.int32,
+                "volume_old": np.float64, "stressyy": np.float64}
+
+
+def read_arrays(path):
+    """Every array of a file in the binary format as raw bytes, by name (an
+    array runs from its offset to the next one's, the last to the end)."""
+    _, _, pos = read_header(path)
+    with open(path, "rb") as f:
+        data = f.read()
+    names = sorted(pos, key=pos.get)
+    ends = [pos[n] for n in names[1:]] + [len(data)]
+    return {n: data[pos[n]:end] for n, end in zip(names, ends)}
+
+
+def read_checkpoint(path):
+    """The arrays of a .chkpt file (Output::write_checkpoint,
+    output.cxx:254-276): the known ones typed, the others (marker sets) as
+    raw bytes."""
+    ndims, _, _ = read_header(path)
+    out = {}
+    for name, raw in read_arrays(path).items():
+        dt = CHKPT_DTYPES.get(name)
+        a = np.frombuffer(raw, dtype=dt).copy() if dt is not None else raw
+        if name == "segment":
+            a = a.reshape(-1, ndims)
+        out[name] = a
+    return out

@@ -238,6 +238,12 @@ int des_apply_vbcs(des_ctx *ctx);           /* bc.cxx:179-490 */
 int des_update_mesh(des_ctx *ctx);          /* dynearthsol.cxx:221-232 */
 int des_rotate_stress(des_ctx *ctx);        /* fields.cxx:411-483 */
 int des_compute_dt(des_ctx *ctx, double *dt_out);   /* geometry.cxx:239-310; sets var.dt */
+/* The tail of remesh() (remeshing.cxx:2006-2029), after the host has built
+   the new mesh, created a context on it and uploaded the interpolated fields:
+   volume, volume_old = volume, mass / thermal mass / nodal volume, shape
+   functions and, for remeshing options 1, 2 and 11, the bottom nodes' coord0
+   reset to -zlength.  dt and the other scalars carry over (des_set_scalars). */
+int des_remesh_finish(des_ctx *ctx, int remeshing_option);
 int des_worst_quality(des_ctx *ctx, double *q, int *worst_elem); /* geometry.cxx:499-512 */
 /* des_worst_quality without allocating device memory: no counterpart in the
    reference; for callers that cannot allocate (HIP graph capture, tight

@@ -108,6 +108,10 @@ def ref_lib(ndims, omp=False, strict=False):
     return lib
 
 
+def ref_io_available(ndims=3):
+    return os.path.exists(os.path.join(HERE, "_ref", f"libref{ndims}d_io.so"))
+
+
 def ref_available(ndims=3, omp=False, strict=False):
     suffix = "_strict" if strict else ("_omp" if omp else "")
     return os.path.exists(os.path.join(HERE, "_ref", f"libref{ndims}d{suffix}.so"))
@@ -324,6 +328,21 @@ class OracleState:
     def ref_call(self, name, omp=False, strict=False):
         lib = ref_lib(self.nd, omp, strict and self.nd == 3)
         rc = getattr(lib, f"ref{self.nd}_call")(C.byref(self.params), C.byref(self.S), REF_CALL[name])
+        assert rc == 0
+        return rc
+
+    def ref_write_output(self, modelname, frame, segment, segflag):
+        """The reference's Output::write and Output::write_checkpoint of this
+        state (oracle/_ref/libref{2,3}d_io.so): <modelname>.save.<frame>,
+        .chkpt.<frame> and a line of <modelname>.info."""
+        lib = _lib(os.path.join("_ref", f"libref{self.nd}d_io.so"))
+        fn = getattr(lib, f"ref{self.nd}_write_output")
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_int, C.c_int, _i, _i]
+        fn.restype = C.c_int
+        seg = np.ascontiguousarray(segment, dtype=np.int32)
+        flg = np.ascontiguousarray(segflag, dtype=np.int32)
+        rc = fn(C.byref(self.params), C.byref(self.S), str(modelname).encode(), frame, len(flg),
+                abi.iptr(seg), abi.iptr(flg))
         assert rc == 0
         return rc
 

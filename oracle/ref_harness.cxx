@@ -387,3 +387,32 @@ extern "C" int ref3_dsyevh3(const double *a6, double *w, double *q9)
     return r;
 }
 #endif
+
+#ifdef DES_REF_IO
+/* Output::write (output.cxx:79-208) and Output::write_checkpoint
+   (output.cxx:254-276) of the reference, on the reference model built from S:
+   "<modelname>.save.<frame>", "<modelname>.chkpt.<frame>" and a line of
+   "<modelname>.info" (frame 0 creates it).  The boundary segments are the
+   caller's (nseg x NDIMS node indices and their flags); no marker sets. */
+#include "output.hpp"
+extern "C" int RFN(write_output)(const des_params *p, ostate *S, const char *modelname, int frame, int nseg,
+                                 const int *segment, const int *segflag)
+{
+    RefModel m(p, S);
+    m.param.sim.modelname = modelname;
+    m.param.sim.is_outputting_averaged_fields = false;
+    m.param.sim.has_marker_output = false;
+    m.param.mesh.quality_check_step_interval = 100;
+    m.var.nseg = nseg;
+    m.var.segment = new segment_t(nseg);
+    m.var.segflag = new segflag_t(nseg);
+    if (nseg) {
+        std::memcpy(m.var.segment->data(), segment, sizeof(int) * nseg * NDIMS);
+        std::memcpy(m.var.segflag->data(), segflag, sizeof(int) * nseg);
+    }
+    Output out(m.param, 0.0, frame);
+    out.write_checkpoint(m.param, m.var);      // before the frame, as the main loop does (dynearthsol.cxx:381-384)
+    out.write(m.var, false);
+    return 0;
+}
+#endif

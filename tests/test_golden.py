@@ -12,7 +12,9 @@ from dynearthsol_amd import abi
 from oracle import OracleState
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FIXTURES = sorted(glob.glob(os.path.join(HERE, "*.npz")))
+# step trajectories (frame_*.npz hold output files; tests/test_golden_frames.py)
+FIXTURES = sorted(p for p in glob.glob(os.path.join(HERE, "*.npz"))
+                  if not os.path.basename(p).startswith("frame_"))
 
 
 def load_fixture(path):
