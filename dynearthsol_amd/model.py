@@ -327,6 +327,42 @@ def box_mesh(ndims, lengths, cells, jitter=0.0, seed=0):
     return _renumber(coord, conn, lengths)
 
 
+def box_delaunay_mesh(ndims, lengths, cells, jitter=0.35, seed=0):
+    """Unstructured box mesh: the Delaunay triangulation (scipy.spatial,
+    Qhull) of a lattice of points jittered by up to `jitter` of a cell along
+    every axis in which the point is not on the boundary, so boundary points
+    stay on their faces.  Node degrees vary as in a mesher's output (a
+    structured Kuhn mesh has the same few degrees everywhere).  Degenerate
+    simplices Qhull leaves on the planar faces are dropped; elements are
+    oriented as the structured mesher orients them, then renumbered as the
+    reference does.  Returns (coord, connectivity)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    axes = [np.linspace(0.0, L, n + 1) for L, n in zip(lengths, cells)]
+    axes[-1] = axes[-1] - lengths[-1]                       # z in [-Lz, 0]
+    grids = np.meshgrid(*axes, indexing="ij")
+    coord = np.stack([g.ravel() for g in grids], axis=1)
+    idx = np.meshgrid(*[np.arange(n + 1) for n in cells], indexing="ij")
+    h = np.array([L / n for L, n in zip(lengths, cells)])
+    for d in range(ndims):
+        free = (idx[d].ravel() > 0) & (idx[d].ravel() < cells[d])
+        coord[:, d] += np.where(free, rng.uniform(-jitter, jitter, len(coord)) * h[d], 0.0)
+    conn = Delaunay(coord).simplices.astype(np.int64)
+    if ndims == 3:
+        vol = _tet_volume(coord, conn)
+        keep = np.abs(vol) > 1e-9 * float(np.prod(h))
+        conn, vol = conn[keep], vol[keep]
+        neg = vol < 0
+    else:
+        x0, x1, x2 = (coord[conn[:, i]] for i in range(3))
+        sa = (x1[:, 0] - x0[:, 0]) * (x2[:, 1] - x0[:, 1]) - (x2[:, 0] - x0[:, 0]) * (x1[:, 1] - x0[:, 1])
+        keep = np.abs(sa) > 1e-9 * float(np.prod(h))
+        conn, sa = conn[keep], sa[keep]
+        neg = sa < 0
+    conn[neg, 1], conn[neg, 2] = conn[neg, 2].copy(), conn[neg, 1].copy()
+    return _renumber(coord, conn, lengths)
+
+
 def box_cells_for(ndims, lengths, nelem_target):
     """Cell counts of a box mesh with about nelem_target elements and
     near-cubic cells."""
@@ -489,7 +525,7 @@ class Model:
 
     def __init__(self, cfg_overrides, ndims, cells=None, nelem_target=None,
                  jitter=0.15, mesh_seed=0, plstrain_noise=None, elem_visc=None,
-                 elem_cohesion=None, temperature="erf", elemmarkers=None):
+                 elem_cohesion=None, temperature="erf", elemmarkers=None, mesh="structured"):
         cfg = resolve_cfg(cfg_overrides)
         self.cfg = cfg
         self.ndims = nd = ndims
@@ -504,7 +540,12 @@ class Model:
         if cells is None:
             cells = box_cells_for(nd, lengths, nelem_target)
         self.cells = cells
-        self.coord, self.conn = box_mesh(nd, lengths, cells, jitter=jitter, seed=mesh_seed)
+        if mesh == "delaunay":
+            self.coord, self.conn = box_delaunay_mesh(nd, lengths, cells, seed=mesh_seed)
+        elif mesh == "structured":
+            self.coord, self.conn = box_mesh(nd, lengths, cells, jitter=jitter, seed=mesh_seed)
+        else:
+            raise ValueError(mesh)
         self.nnode, self.nelem = len(self.coord), len(self.conn)
         self.bcflag = box_bcflag(self.coord, lengths)
         self.bfacets = boundary_facets(self.conn, self.bcflag, nd)
