@@ -56,6 +56,8 @@ ELEM_PARAM_BYTES = {"fp64": 16, "mixed": 8}     # per-element viscosity and cohe
 # (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved): (2 FETCH + WRITE)
 # * 1024.  The commands and per-kernel tables are in profiles/.
 TRAFFIC = {
+    "3d-8m": {"k_pass_a": 736113164, "k_pass_b": 1948511891, "k_pass_c": 2504874432, "k_pass_d": 644954136,
+              "k_pass_e": 2002102816, "k_pass_f": 1262318090},
     "3d-fp64": {"k_pass_a": 92690817, "k_pass_b": 225450346, "k_pass_c": 302206372, "k_pass_d": 35101817,
                 "k_pass_e": 239278641, "k_pass_f": 154240694},
 }

@@ -593,8 +593,8 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevS
 
 // node part of update_force: gather, stress BCs, elastic foundation,
 // damping (fields.cxx:291-340, bc.cxx:493-566, fields.cxx:235-288)
-template <int ND, bool F32 = false> __device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND],
-                                                                             const double v[ND])
+template <int ND, bool F32 = false, int DEPTH = 4>
+__device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND], const double v[ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
     const des_params *P = s.P;
@@ -605,14 +605,14 @@ template <int ND, bool F32 = false> __device__ __forceinline__ void node_force(c
         const float *fw = reinterpret_cast<const float *>(s.fwork);
         const int p1 = s.csr_off[n + 1];
         int p = s.csr_off[n];
-        for (; p + 4 <= p1; p += 4) {
-            float c[4][ND];
+        for (; p + DEPTH <= p1; p += DEPTH) {
+            float c[DEPTH][ND];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < DEPTH; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) c[j][d] = fw[(long)(p + j) * ND + d];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < DEPTH; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) f[d] -= (double)c[j][d];
         }
@@ -624,14 +624,14 @@ template <int ND, bool F32 = false> __device__ __forceinline__ void node_force(c
     } else {
         const int p1 = s.csr_off[n + 1];
         int p = s.csr_off[n];
-        for (; p + 4 <= p1; p += 4) {   // four entries in flight, summed in order
-            double c[4][ND];
+        for (; p + DEPTH <= p1; p += DEPTH) {   // DEPTH entries in flight, summed in order
+            double c[DEPTH][ND];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < DEPTH; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) c[j][d] = s.fwork[(long)(p + j) * ND + d];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < DEPTH; ++j)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) f[d] -= c[j][d];
         }
@@ -1246,7 +1246,7 @@ template <int K> __device__ __forceinline__ double quad_bcast(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
+template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0 && s.ret_count) *s.ret_count = 0;   // pass C appends
     // four lanes (a quad) per node: the quad loads an element's 64-byte
@@ -1266,19 +1266,25 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, 
     const int thermal = s.P->has_thermal_diffusion;
     double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
     const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
-    for (int p = p0; p < p1; p += 4) {
-        const int mine = p + q < p1 ? s.csr_ent[p + q] : -1;
-        int en[4];
-        en[0] = quad_bcast<0>(mine);
-        en[1] = quad_bcast<1>(mine);
-        en[2] = quad_bcast<2>(mine);
-        en[3] = quad_bcast<3>(mine);
-        double2 rq[4];
+    // DEPTH entries per iteration: their records are all requested before
+    // the first is summed
+    for (int p = p0; p < p1; p += DEPTH) {
+        int en[DEPTH];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int g = 0; g < DEPTH / 4; ++g) {
+            const int pp = p + 4 * g + q;
+            const int mine = pp < p1 ? s.csr_ent[pp] : -1;
+            en[4 * g + 0] = quad_bcast<0>(mine);
+            en[4 * g + 1] = quad_bcast<1>(mine);
+            en[4 * g + 2] = quad_bcast<2>(mine);
+            en[4 * g + 3] = quad_bcast<3>(mine);
+        }
+        double2 rq[DEPTH];
+#pragma unroll
+        for (int j = 0; j < DEPTH; ++j)
             if (en[j] >= 0) rq[j] = reinterpret_cast<const double2 *>(s.erec + (long)(en[j] / NPE) * REC)[q];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < DEPTH; ++j) {
             if (en[j] < 0) break;
             const int i = en[j] % NPE;
             const double sel = (i & 1) ? rq[j].y : rq[j].x;
@@ -1464,22 +1470,22 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
 }
 
 // pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
-template <int ND, int NB> __global__ __launch_bounds__(NB) void k_pass_d(DevState s, FusedFlags fl)
+template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_d(DevState s, FusedFlags fl)
 {
     BIND_NODE_NB(s, n, NB);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
     const int p1 = s.csr_off[n + 1];
     int p = s.csr_off[n];
-    for (; p + 4 <= p1; p += 4) {       // four entries in flight, summed in order
-        int en[4];
-        double v[4];
+    for (; p + DEPTH <= p1; p += DEPTH) {       // DEPTH entries in flight, summed in order
+        int en[DEPTH];
+        double v[DEPTH];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) en[j] = s.csr_ent[p + j];
+        for (int j = 0; j < DEPTH; ++j) en[j] = s.csr_ent[p + j];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = s.escal[en[j] / NPE];
+        for (int j = 0; j < DEPTH; ++j) v[j] = s.escal[en[j] / NPE];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) sum += v[j];
+        for (int j = 0; j < DEPTH; ++j) sum += v[j];
     }
     for (; p < p1; ++p) sum += s.escal[s.csr_ent[p] / NPE];
     const double dpn = sum / s.volume_n[n];
@@ -1512,8 +1518,8 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(
     store_force_contrib<ND>(s, e, out);
 }
 
-template <int ND, bool F32> __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl, int n,
-                                                                     int tb_)
+template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl,
+                                                                                int n, int tb_)
 {
     const double dt = s.sc->dt;
     // the lane of each model's first node ends the step; nothing in this pass
@@ -1524,7 +1530,7 @@ template <int ND, bool F32> __device__ __forceinline__ void pass_f_body(DevState
     double f[ND], v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = __builtin_nontemporal_load(s.vel + d * s.N + n);
-    node_force<ND, F32>(s, n, f, v);
+    node_force<ND, F32, DEPTH>(s, n, f, v);
     if (fl.store_aux) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
@@ -1550,17 +1556,17 @@ template <int ND, bool F32> __device__ __forceinline__ void pass_f_body(DevState
     }
 }
 
-template <int ND, int NB> __global__ __launch_bounds__(NB) void k_pass_f(DevState s, FusedFlags fl)
+template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f(DevState s, FusedFlags fl)
 {
     BIND_NODE_NB(s, n, NB);
-    pass_f_body<ND, false>(s, fl, n, tb_);
+    pass_f_body<ND, false, DEPTH>(s, fl, n, tb_);
 }
 
 // pass F of the mixed-precision schedule: FP32 force contributions
-template <int ND, int NB> __global__ __launch_bounds__(NB) void k_pass_f32(DevState s, FusedFlags fl)
+template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f32(DevState s, FusedFlags fl)
 {
     BIND_NODE_NB(s, n, NB);
-    pass_f_body<ND, true>(s, fl, n, tb_);
+    pass_f_body<ND, true, DEPTH>(s, fl, n, tb_);
 }
 
 // ---- mixed precision (DES_MIXED): passes C and E on (pres, sdev) -------
@@ -1835,9 +1841,9 @@ struct des_ctx {
     std::vector<int> h_conn;       // host copies for output frames
     std::vector<unsigned> h_bcflag;
     // the autotuner's choice of variant per tunable fused pass (DES_TUNE_*
-    // order: A, C, D, E, F), per precision (0 FP64, 1 mixed); -1 untuned
-    int tune_var[2][DES_NTUNE] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
-    double tune_ms[2][DES_NTUNE][4] = {};   // mean ms per variant it measured
+    // order: A, C, D, E, F, B), per precision (0 FP64, 1 mixed); -1 untuned
+    int tune_var[2][DES_NTUNE] = {{-1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1}};
+    double tune_ms[2][DES_NTUNE][DES_TUNE_MAXV] = {};   // mean ms per variant it measured
     struct TuneLaunch { int pass, variant; hipEvent_t a, b; };
     std::vector<TuneLaunch> tune;          // timed tuning launches in flight
     // batched group (nmodels > 0): models of one dimension concatenated, each
@@ -1976,36 +1982,43 @@ int ensure_xfer(des_ctx *ctx, size_t n)
 
 // Transfers in the host's numbering: the rows are permuted and interleaved
 // on the device, and one copy crosses PCIe.
+// src_dev: src is device memory (host row order, AoS) and is read in place
 int upload_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0,
-               const int *perm = nullptr)
+               const int *perm = nullptr, bool src_dev = false)
 {
     if (!src || n <= 0) return DES_OK;
     if (stride < 0) stride = n;
     const int *dp = device_perm(ctx, perm);
     if (perm && !dp) return fail(ctx, DES_ERR_INPUT, "upload: unknown row order");
     int rc;
-    if ((rc = ensure_xfer(ctx, (size_t)n * k))) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->xfer, src, sizeof(double) * n * k, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblocks(n)), dim3(BLOCK), 0, ctx->stream, ctx->xfer, n, k, dp,
+    const double *aos = src;
+    if (!src_dev) {
+        if ((rc = ensure_xfer(ctx, (size_t)n * k))) return rc;
+        HIPCHK(hipMemcpyAsync(ctx->xfer, src, sizeof(double) * n * k, hipMemcpyHostToDevice, ctx->stream));
+        aos = ctx->xfer;
+    }
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblocks(n)), dim3(BLOCK), 0, ctx->stream, aos, n, k, dp,
                        dst + off, stride);
     if ((rc = check_launch(ctx))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
 }
 
+// dst_dev: dst is device memory (host row order, AoS) written in place
 int download_soa(des_ctx *ctx, double *dst, const double *src, long n, int k, long stride = -1, long off = 0,
-                 const int *perm = nullptr)
+                 const int *perm = nullptr, bool dst_dev = false)
 {
     if (!dst || n <= 0) return DES_OK;
     if (stride < 0) stride = n;
     const int *dp = device_perm(ctx, perm);
     if (perm && !dp) return fail(ctx, DES_ERR_INPUT, "download: unknown row order");
     int rc;
-    if ((rc = ensure_xfer(ctx, (size_t)n * k))) return rc;
+    if (!dst_dev && (rc = ensure_xfer(ctx, (size_t)n * k))) return rc;
     hipLaunchKernelGGL(k_soa_to_aos, dim3(nblocks(n)), dim3(BLOCK), 0, ctx->stream, src + off, stride, n, k, dp,
-                       ctx->xfer);
+                       dst_dev ? dst : ctx->xfer);
     if ((rc = check_launch(ctx))) return rc;
-    HIPCHK(hipMemcpyAsync(dst, ctx->xfer, sizeof(double) * n * k, hipMemcpyDeviceToHost, ctx->stream));
+    if (!dst_dev)
+        HIPCHK(hipMemcpyAsync(dst, ctx->xfer, sizeof(double) * n * k, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return DES_OK;
 }
@@ -2650,19 +2663,20 @@ void des_destroy(des_ctx *ctx)
 namespace {
 
 // per-element optional field, allocated on first upload
-int field_opt(des_ctx *ctx, double **slot, const double *src, long n, long off)
+int field_opt(des_ctx *ctx, double **slot, const double *src, long n, long off, bool dev = false)
 {
     if (!src) return DES_OK;
     if (!*slot) {
         int rc = dalloc_n(ctx, slot, (size_t)ctx->E);
         if (rc) return rc;
     }
-    return upload_soa(ctx, *slot, src, n, 1, ctx->E, off, ctx->eh.data() + off);
+    return upload_soa(ctx, *slot, src, n, 1, ctx->E, off, ctx->eh.data() + off, dev);
 }
 
-// The fields of des_fields between host AoS arrays and the device SoA state:
-// the whole context (model < 0) or batched model `model` of a group.
-int xfer_fields(des_ctx *ctx, des_fields *f, bool up, int model)
+// The fields of des_fields between AoS arrays in host row order and the
+// device SoA state: the whole context (model < 0) or batched model `model` of
+// a group.  dev: the des_fields pointers are device memory.
+int xfer_fields(des_ctx *ctx, des_fields *f, bool up, int model, bool dev = false)
 {
     int rc = flush_deferred(ctx);
     if (rc) return rc;
@@ -2686,22 +2700,22 @@ int xfer_fields(des_ctx *ctx, des_fields *f, bool up, int model)
     for (auto &t : tab) {
         const long stride = t.elem ? ES : NS, off = t.elem ? eo : no;
         const int *perm = (t.elem ? ctx->eh.data() : ctx->nh.data()) + off;
-        rc = up ? upload_soa(ctx, t.dev, t.host, t.n, t.k, stride, off, perm)
-                : download_soa(ctx, t.host, t.dev, t.n, t.k, stride, off, perm);
+        rc = up ? upload_soa(ctx, t.dev, t.host, t.n, t.k, stride, off, perm, dev)
+                : download_soa(ctx, t.host, t.dev, t.n, t.k, stride, off, perm, dev);
         if (rc) return rc;
     }
     double *shp_host[3] = {f->shpdx, nd == 3 ? f->shpdy : f->shpdz, nd == 3 ? f->shpdz : nullptr};
     for (int d = 0; d < nd; ++d) {
-        double *dev = s.shp + (size_t)d * npe * ES;
+        double *shpd = s.shp + (size_t)d * npe * ES;
         const int *perm = ctx->eh.data() + eo;
-        rc = up ? upload_soa(ctx, dev, shp_host[d], E, npe, ES, eo, perm)
-                : download_soa(ctx, shp_host[d], dev, E, npe, ES, eo, perm);
+        rc = up ? upload_soa(ctx, shpd, shp_host[d], E, npe, ES, eo, perm, dev)
+                : download_soa(ctx, shp_host[d], shpd, E, npe, ES, eo, perm, dev);
         if (rc) return rc;
     }
     if (up) {
         double *ev = const_cast<double *>(s.elem_visc), *ec = const_cast<double *>(s.elem_cohesion);
-        if ((rc = field_opt(ctx, &ev, f->elem_visc, E, eo))) return rc;
-        if ((rc = field_opt(ctx, &ec, f->elem_cohesion, E, eo))) return rc;
+        if ((rc = field_opt(ctx, &ev, f->elem_visc, E, eo, dev))) return rc;
+        if ((rc = field_opt(ctx, &ec, f->elem_cohesion, E, eo, dev))) return rc;
         s.elem_visc = ev;
         s.elem_cohesion = ec;
         if (model >= 0) {
@@ -2721,6 +2735,21 @@ int des_upload_fields(des_ctx *ctx, const des_fields *f)
 {
     if (!ctx || !f) return DES_ERR_INPUT;
     return xfer_fields(ctx, const_cast<des_fields *>(f), true, -1);
+}
+
+int des_upload_fields_device(des_ctx *ctx, const des_fields *f)
+{
+    if (!ctx || !f) return DES_ERR_INPUT;
+    if (ctx->nmodels) return fail(ctx, DES_ERR_INPUT, "device-pointer transfers are for single models");
+    des_fields c = *f;
+    return xfer_fields(ctx, &c, true, -1, true);
+}
+
+int des_download_fields_device(des_ctx *ctx, des_fields *f)
+{
+    if (!ctx || !f) return DES_ERR_INPUT;
+    if (ctx->nmodels) return fail(ctx, DES_ERR_INPUT, "device-pointer transfers are for single models");
+    return xfer_fields(ctx, f, false, -1, true);
 }
 
 int des_download_fields(des_ctx *ctx, des_fields *f)
@@ -3368,15 +3397,17 @@ int ensure_mixed(des_ctx *ctx)
 //   C:    a minimum of 1, 2, 3 or 4 waves per SIMD (__launch_bounds__): fewer
 //         registers per lane buy occupancy at the price of spills (FP64 only;
 //         the mixed pass C has one variant);
-//   D, F: node blocks of 256, 128 or 64 threads.
+//   D, F: node blocks of 256, 128 or 64 threads; D also 256 threads with 8
+//         instead of 4 gathered entries in flight;
+//   B:    4 or 8 element records in flight per node.
 // The first TUNE_STEPS steps of a run of at least TUNE_MIN_RUN steps time
 // every variant of every untuned pass with HIP events (variant k mod the
 // variant count at tuning step k) and keep the fastest mean.
 constexpr int TUNE_A = DES_TUNE_A, TUNE_C = DES_TUNE_C, TUNE_D = DES_TUNE_D, TUNE_E = DES_TUNE_E,
-              TUNE_F = DES_TUNE_F;
-constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 4, 3, 2, 3}, {2, 1, 3, 2, 3}};
-constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 0, 0, 0};
-constexpr int TUNE_STEPS = 8, TUNE_MIN_RUN = 16;
+              TUNE_F = DES_TUNE_F, TUNE_B = DES_TUNE_B;
+constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 4, 5, 2, 5, 3}, {2, 1, 5, 2, 5, 3}};
+constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 3, 0, 3, 1};
+constexpr int TUNE_STEPS = 2 * DES_TUNE_MAXV - 2, TUNE_MIN_RUN = 16;
 constexpr int PASSC_VARIANTS = 4;
 
 void clear_tuning_events(des_ctx *ctx)
@@ -3414,12 +3445,43 @@ int launch_pass_a(des_ctx *ctx, const FusedFlags &fl, int variant)
     return DES_OK;
 }
 
+// kernel<ND, NB, DEPTH> in blocks of NB threads over nthreads threads
+#define LAUNCH_T3(kern, NB, DEPTH, nthreads, ...)                                                   \
+    do {                                                                                          \
+        const dim3 g_((unsigned)(((nthreads) + (NB) - 1) / (NB))), b_(NB);                        \
+        if (ctx->nd == 3)                                                                         \
+            hipExtLaunchKernelGGL((kern<3, NB, DEPTH>), g_, b_, 0, ctx->stream, ctx->ev_a,        \
+                                  ctx->ev_b, 0, __VA_ARGS__);                                     \
+        else                                                                                      \
+            hipExtLaunchKernelGGL((kern<2, NB, DEPTH>), g_, b_, 0, ctx->stream, ctx->ev_a,        \
+                                  ctx->ev_b, 0, __VA_ARGS__);                                     \
+        int rc_ = check_launch(ctx);                                                              \
+        if (rc_) return rc_;                                                                      \
+    } while (0)
+
+int launch_pass_b(des_ctx *ctx, const FusedFlags &fl, int variant)
+{
+    const long nthreads = 4L * ctx->N;
+    if (variant == 2) LAUNCH_T2(k_pass_b, 16, BLOCK, nblocks(nthreads), ctx->s, fl);
+    else if (variant == 1) LAUNCH_T2(k_pass_b, 8, BLOCK, nblocks(nthreads), ctx->s, fl);
+    else LAUNCH_T2(k_pass_b, 4, BLOCK, nblocks(nthreads), ctx->s, fl);
+    return DES_OK;
+}
+
+// the node passes D and F: blocks of 256/128/64 threads with 4 entries in
+// flight (variants 0..2), 256 threads with 8 or 16 in flight (3, 4)
+#define NODE_PASS_VARIANTS(kern, ...)                                                             \
+    switch (variant) {                                                                            \
+    case 1: LAUNCH_T3(kern, 128, 4, ctx->N, __VA_ARGS__); break;                                  \
+    case 2: LAUNCH_T3(kern, 64, 4, ctx->N, __VA_ARGS__); break;                                   \
+    case 3: LAUNCH_T3(kern, 256, 8, ctx->N, __VA_ARGS__); break;                                  \
+    case 4: LAUNCH_T3(kern, 256, 16, ctx->N, __VA_ARGS__); break;                                 \
+    default: LAUNCH_T3(kern, 256, 4, ctx->N, __VA_ARGS__); break;                                 \
+    }
+
 int launch_pass_d(des_ctx *ctx, const FusedFlags &fl, int variant)
 {
-    const long N = ctx->N;
-    if (variant == 2) LAUNCH_T2(k_pass_d, 64, 64, (N + 63) / 64, ctx->s, fl);
-    else if (variant == 1) LAUNCH_T2(k_pass_d, 128, 128, (N + 127) / 128, ctx->s, fl);
-    else LAUNCH_T2(k_pass_d, 256, 256, (N + 255) / 256, ctx->s, fl);
+    NODE_PASS_VARIANTS(k_pass_d, ctx->s, fl);
     return DES_OK;
 }
 
@@ -3437,15 +3499,10 @@ int launch_pass_e(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
 
 int launch_pass_f(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
 {
-    const long N = ctx->N;
     if (mixed) {
-        if (variant == 2) LAUNCH_T2(k_pass_f32, 64, 64, (N + 63) / 64, ctx->s, fl);
-        else if (variant == 1) LAUNCH_T2(k_pass_f32, 128, 128, (N + 127) / 128, ctx->s, fl);
-        else LAUNCH_T2(k_pass_f32, 256, 256, (N + 255) / 256, ctx->s, fl);
+        NODE_PASS_VARIANTS(k_pass_f32, ctx->s, fl);
     } else {
-        if (variant == 2) LAUNCH_T2(k_pass_f, 64, 64, (N + 63) / 64, ctx->s, fl);
-        else if (variant == 1) LAUNCH_T2(k_pass_f, 128, 128, (N + 127) / 128, ctx->s, fl);
-        else LAUNCH_T2(k_pass_f, 256, 256, (N + 255) / 256, ctx->s, fl);
+        NODE_PASS_VARIANTS(k_pass_f, ctx->s, fl);
     }
     return DES_OK;
 }
@@ -3560,7 +3617,7 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     if ((rc = tunable(TUNE_A, [&](int v) { return launch_pass_a(ctx, fl, v); }))) return rc;
     MARK(0);
     ARM(false);
-    LAUNCH_ND(k_pass_b, nblocks(4L * ctx->N), ctx->s, fl);
+    if ((rc = tunable(TUNE_B, [&](int v) { return launch_pass_b(ctx, fl, v); }))) return rc;
     MARK(1);
     ARM(false);
     rc = tunable(TUNE_C, [&](int v) -> int {
@@ -3596,8 +3653,8 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     if (tuning && k == TUNE_STEPS - 1 && !ctx->tune.empty()) {
         // the fastest mean per untuned pass
         HIPCHK(hipEventSynchronize(ctx->tune.back().b));
-        double sum[DES_NTUNE][4] = {};
-        int cnt[DES_NTUNE][4] = {};
+        double sum[DES_NTUNE][DES_TUNE_MAXV] = {};
+        int cnt[DES_NTUNE][DES_TUNE_MAXV] = {};
         for (auto &t : ctx->tune) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
@@ -3661,14 +3718,15 @@ int des_set_tuning(des_ctx *ctx, int variant)
     return DES_OK;
 }
 
-int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE], double ms[4 * DES_NTUNE])
+int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE],
+                       double ms[DES_TUNE_MAXV * DES_NTUNE])
 {
     if (!ctx || !variant || (precision != DES_FP64 && precision != DES_MIXED)) return DES_ERR_INPUT;
     const int pr = precision == DES_MIXED ? 1 : 0;
     for (int p = 0; p < DES_NTUNE; ++p) {
         variant[p] = ctx->tune_var[pr][p];
         if (ms)
-            for (int v = 0; v < 4; ++v) ms[4 * p + v] = ctx->tune_ms[pr][p][v];
+            for (int v = 0; v < DES_TUNE_MAXV; ++v) ms[DES_TUNE_MAXV * p + v] = ctx->tune_ms[pr][p][v];
     }
     return DES_OK;
 }
@@ -4061,7 +4119,8 @@ int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4])
     return bfail(b, DES_ERR_INPUT, "no group of that dimension");
 }
 
-int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES_NTUNE], double ms[4 * DES_NTUNE])
+int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES_NTUNE],
+                         double ms[DES_TUNE_MAXV * DES_NTUNE])
 {
     if (!b || !variant) return DES_ERR_INPUT;
     for (des_ctx *ctx : b->groups)

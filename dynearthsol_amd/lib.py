@@ -30,15 +30,18 @@ EXPORTS = (
     "des_batch_tuning", "des_adj_forward", "des_adj_backward", "des_adj_release", "des_set_tuning",
     "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
     "des_worst_quality_noalloc", "des_get_tuning_all", "des_set_tuning_all", "des_batch_tuning_all",
-    "des_write_checkpoint", "des_remesh_finish",
+    "des_write_checkpoint", "des_remesh_finish", "des_upload_fields_device", "des_download_fields_device",
+    "des_adj_backward_device",
 )
 
-NTUNE = 5
-TUNE_PASSES = ("pass_a", "pass_c", "pass_d", "pass_e", "pass_f")
+NTUNE = 6
+TUNE_MAXV = 6
+TUNE_PASSES = ("pass_a", "pass_c", "pass_d", "pass_e", "pass_f", "pass_b")
+_NODE_VARIANTS = ("block256-depth4", "block128-depth4", "block64-depth4", "block256-depth8", "block256-depth16")
 TUNE_MEANING = {
     "pass_a": ("lds-staged", "direct"), "pass_e": ("lds-staged", "direct"),
     "pass_c": ("waves>=1", "waves>=2", "waves>=3", "waves>=4"),
-    "pass_d": ("block256", "block128", "block64"), "pass_f": ("block256", "block128", "block64"),
+    "pass_d": _NODE_VARIANTS, "pass_f": _NODE_VARIANTS, "pass_b": ("depth4", "depth8", "depth16"),
 }
 
 PER_FUNCTION = (
@@ -64,6 +67,18 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run python -m dynearthsol_amd.build")
+    # One HIP runtime per process.  torch's wheel carries its own
+    # libamdhip64 / libhsa-runtime64; a second runtime in the same process
+    # cannot open the GPU once the first has (the GPU's address space is
+    # held by one runtime's device file), so whichever initialises second
+    # sees no device.  torch's copies answer to the sonames this library
+    # needs (libamdhip64.so.7, libhsa-runtime64.so.1): with torch loaded
+    # first the dynamic linker binds this library to torch's runtime, and
+    # device pointers pass between the two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     lib.des_create.argtypes = [C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
@@ -90,6 +105,9 @@ def load():
     lib.des_write_frame.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     lib.des_write_frame_averaged.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     lib.des_remesh_finish.argtypes = [vp, C.c_int]
+    lib.des_upload_fields_device.argtypes = [vp, C.POINTER(abi.DesFields)]
+    lib.des_download_fields_device.argtypes = [vp, C.POINTER(abi.DesFields)]
+    lib.des_adj_backward_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.des_write_checkpoint.argtypes = [vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.c_int, C.POINTER(abi.DesNamedArray)]
     lib.des_set_averaging.argtypes = [vp, C.c_int]
@@ -304,15 +322,38 @@ class DesModel:
         self._check(self.lib.des_adj_backward(self.ctx, abi.dptr(cb), abi.dptr(vb), abi.dptr(hb)))
         return vb, hb
 
+    def upload_device(self, **ptrs):
+        """des_upload_fields_device: fields given as device addresses (int)
+        of arrays in the host's row order and AoS layout."""
+        f = abi.DesFields()
+        for k, v in ptrs.items():
+            if v is not None:
+                setattr(f, k, C.cast(C.c_void_p(int(v)), C.POINTER(C.c_double)))
+        self._check(self.lib.des_upload_fields_device(self.ctx, C.byref(f)))
+
+    def download_device(self, **ptrs):
+        """des_download_fields_device into device addresses (int)."""
+        f = abi.DesFields()
+        for k, v in ptrs.items():
+            if v is not None:
+                setattr(f, k, C.cast(C.c_void_p(int(v)), C.POINTER(C.c_double)))
+        self._check(self.lib.des_download_fields_device(self.ctx, C.byref(f)))
+
+    def adj_backward_device(self, coord_bar, visc_bar, cohesion_bar):
+        """des_adj_backward_device on device addresses (int): coord_bar
+        (nnode x ndims) in, visc_bar and cohesion_bar (nelem) out."""
+        self._check(self.lib.des_adj_backward_device(self.ctx, C.c_void_p(int(coord_bar)),
+                                                     C.c_void_p(int(visc_bar)), C.c_void_p(int(cohesion_bar))))
+
     def adj_release(self):
         self.lib.des_adj_release(self.ctx)
 
     def tuning_all(self, precision=abi.DES_FP64):
         """{pass: (variant or -1, [mean ms per variant])} of the autotuner."""
         v = (C.c_int * NTUNE)()
-        ms = (C.c_double * (4 * NTUNE))()
+        ms = (C.c_double * (TUNE_MAXV * NTUNE))()
         self._check(self.lib.des_get_tuning_all(self.ctx, precision, v, ms))
-        return {p: (v[k], list(ms[4 * k:4 * k + 4])) for k, p in enumerate(TUNE_PASSES)}
+        return {p: (v[k], list(ms[TUNE_MAXV * k:TUNE_MAXV * (k + 1)])) for k, p in enumerate(TUNE_PASSES)}
 
     def set_tuning_all(self, precision, choices):
         """Fix the autotuner's choices ({pass: variant} or a sequence in
@@ -459,9 +500,9 @@ class DesBatch:
     def tuning_all(self, ndims, precision=abi.DES_FP64):
         """{pass: (variant or -1, [mean ms per variant])} of the group of that dimension."""
         v = (C.c_int * NTUNE)()
-        ms = (C.c_double * (4 * NTUNE))()
+        ms = (C.c_double * (TUNE_MAXV * NTUNE))()
         self._check(self.lib.des_batch_tuning_all(self.b, ndims, precision, v, ms))
-        return {p: (v[k], list(ms[4 * k:4 * k + 4])) for k, p in enumerate(TUNE_PASSES)}
+        return {p: (v[k], list(ms[TUNE_MAXV * k:TUNE_MAXV * (k + 1)])) for k, p in enumerate(TUNE_PASSES)}
 
     def download(self, k, names=None):
         out, f = _field_buffers(self.models[k], names)

@@ -168,6 +168,13 @@ int des_upload_fields(des_ctx *ctx, const des_fields *f);
 /* Completes any deferred work of the fused driver first, so downloaded
    arrays equal the reference's Variables after the same number of steps. */
 int des_download_fields(des_ctx *ctx, des_fields *f);
+/* The same with every des_fields pointer in device memory of the context's
+   GPU (still the host's row order and AoS layout): no host copy, for callers
+   whose arrays live on the device (a torch.autograd binding's tensors).
+   Single-model contexts; the context's stream is synchronized on return, so
+   the caller's own stream must have finished writing the inputs. */
+int des_upload_fields_device(des_ctx *ctx, const des_fields *f);
+int des_download_fields_device(des_ctx *ctx, des_fields *f);
 
 /* Output::write (output.cxx:79-208) of the instantaneous fields: writes
    "<modelname>.save.<frame %06d>" in the reference's binary format
@@ -291,21 +298,29 @@ int des_sync(des_ctx *ctx);
                              1 gathered directly;
      DES_TUNE_C:             FP64 pass C with a minimum of 1..4 waves per SIMD
                              (0..3); the mixed pass C has one variant (0);
-     DES_TUNE_D, DES_TUNE_F: node blocks of 256, 128 or 64 threads (0..2).
-   The first 8 steps of a run of 16 or more steps time every variant of each
-   pass not tuned yet (variant k mod the variant count at step k) and keep the
-   fastest mean; FP64 and mixed-precision runs tune separately, and each
-   dimension's group of a batch tunes on its own. */
-#define DES_NTUNE 5
+     DES_TUNE_D, DES_TUNE_F: node blocks of 256, 128 or 64 threads with 4
+                             gathered entries in flight per node (0..2), or
+                             256 threads with 8 or 16 in flight (3, 4);
+     DES_TUNE_B:             4, 8 or 16 element records in flight per node
+                             (0..2).
+   Until tuned a pass runs variant 0, except D and F (3) and B (1).  The first
+   10 steps of a run of 16 or more steps time every variant of each pass not
+   tuned yet (variant k mod the variant count at step k) and keep the fastest
+   mean; FP64 and mixed-precision runs tune separately, and each dimension's
+   group of a batch tunes on its own. */
+#define DES_NTUNE 6
 #define DES_TUNE_A 0
 #define DES_TUNE_C 1
 #define DES_TUNE_D 2
 #define DES_TUNE_E 3
 #define DES_TUNE_F 4
+#define DES_TUNE_B 5
+#define DES_TUNE_MAXV 6     /* variants per pass at most */
 /* variant[p]: the choice for pass p at this precision (DES_FP64 or DES_MIXED),
-   -1 while untuned; ms[4 * p + v]: the mean milliseconds measured for
-   variant v of pass p (0 where not measured). */
-int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE], double ms[4 * DES_NTUNE]);
+   -1 while untuned; ms[DES_TUNE_MAXV * p + v]: the mean milliseconds measured
+   for variant v of pass p (0 where not measured). */
+int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE],
+                       double ms[DES_TUNE_MAXV * DES_NTUNE]);
 /* Fix the choices (each -1 to let the next long run tune that pass). */
 int des_set_tuning_all(des_ctx *ctx, int precision, const int variant[DES_NTUNE]);
 /* The FP64 pass-C entry of the above: variant 0..3 or -1, ms per variant. */
@@ -339,6 +354,9 @@ int des_adj_forward(des_ctx *ctx, int nsteps, int window);
    the min / max) and cohesion_bar[e] = d loss / d elem_cohesion[e] (either
    may be null).  The context is left at the final state of the run. */
 int des_adj_backward(des_ctx *ctx, const double *coord_bar, double *visc_bar, double *cohesion_bar);
+/* des_adj_backward with coord_bar, visc_bar and cohesion_bar in device memory
+   (host row order). */
+int des_adj_backward_device(des_ctx *ctx, const double *coord_bar, double *visc_bar, double *cohesion_bar);
 
 /* Frees the checkpoints and the reverse sweep's buffers. */
 void des_adj_release(des_ctx *ctx);
@@ -373,7 +391,7 @@ int des_batch_sync(des_batch *b);
 int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem);   /* per model */
 int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4]);
 int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES_NTUNE],
-                         double ms[4 * DES_NTUNE]);
+                         double ms[DES_TUNE_MAXV * DES_NTUNE]);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ def test_header_declares_the_reference_functions():
 
 @pytest.mark.skipif(not os.path.exists(lib.LIB_PATH), reason="library not built")
 def test_library_exports_every_declared_symbol():
-    h = C.CDLL(lib.LIB_PATH)
+    h = lib.load()
     missing = [n for n in _declared() if not hasattr(h, n)]
     assert not missing, missing
     assert set(lib.EXPORTS) <= set(_declared())
@@ -43,3 +43,16 @@ def test_struct_layouts_match_header():
                 if line.strip().startswith("double"))
     assert n_ptr == len(abi.FIELD_NAMES) == len(abi.DesFields._fields_)
     assert C.sizeof(abi.DesFields) == 8 * n_ptr
+
+
+def test_library_and_torch_share_one_hip_runtime():
+    """Loaded through the package, the library binds to the HIP runtime
+    torch carries: one libamdhip64 and one libhsa-runtime64 mapped in the
+    process (a second runtime could not open the GPU the first holds)."""
+    pytest.importorskip("torch")
+    lib.load()
+    import torch  # noqa: F401
+    maps = open("/proc/self/maps").read().split("\n")
+    for stem in ("libamdhip64", "libhsa-runtime64"):
+        paths = {ln.split()[-1] for ln in maps if stem in ln}
+        assert len(paths) == 1, paths

@@ -166,12 +166,15 @@ def test_oversized_tape_is_refused():
     assert ex.value.code == 101
 
 
-def test_torch_autograd_binding():
+@pytest.mark.parametrize("dev", ["cuda", "cpu"])
+def test_torch_autograd_binding(dev):
+    """Device tensors (passed to the library as device pointers) and host
+    tensors (host copies) give the C-ABI's gradient."""
     import torch
     from dynearthsol_amd.autograd import SurfaceTopography
     m, _ = _case("maxwell")
     topo = SurfaceTopography(m, nsteps=10, window=4)
-    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    assert torch.cuda.is_available(), "torch sees no GPU next to the library"
     lv = torch.tensor(np.log10(m.elem_visc), dtype=torch.float64, device=dev, requires_grad=True)
     coh = torch.tensor(m.elem_cohesion, dtype=torch.float64, device=dev, requires_grad=True)
     uz = topo(lv, coh)
@@ -205,7 +208,8 @@ def test_torch_binding_refuses_a_stale_graph():
     from dynearthsol_amd.autograd import SurfaceTopography
     m, _ = _case("maxwell")
     topo = SurfaceTopography(m, nsteps=6, window=3)
-    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    assert torch.cuda.is_available()
+    dev = "cuda"
     lv = torch.tensor(np.log10(m.elem_visc), dtype=torch.float64, device=dev, requires_grad=True)
     coh = torch.tensor(m.elem_cohesion, dtype=torch.float64, device=dev, requires_grad=True)
     first = topo(lv, coh).sum()
@@ -214,3 +218,35 @@ def test_torch_binding_refuses_a_stale_graph():
         first.backward()
     second.backward()          # the latest graph still works
     assert torch.isfinite(lv.grad).all()
+
+
+def test_device_pointer_entry_points_equal_host_ones():
+    """des_upload_fields_device / des_download_fields_device /
+    des_adj_backward_device on torch device tensors give bitwise what the
+    host-pointer entry points give."""
+    import torch
+    from dynearthsol_amd.lib import DesModel
+    m, cb = _case("maxwell")
+    assert torch.cuda.is_available()
+    dev = torch.device("cuda")
+    g = DesModel(m)
+    h = DesModel(m)
+    v = torch.tensor(m.elem_visc * 1.5, dtype=torch.float64, device=dev)
+    c = torch.tensor(m.elem_cohesion * 0.8, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    g.upload_device(elem_visc=v.data_ptr(), elem_cohesion=c.data_ptr())
+    h.upload(elem_visc=m.elem_visc * 1.5, elem_cohesion=m.elem_cohesion * 0.8)
+    g.adj_forward(8, 3)
+    h.adj_forward(8, 3)
+    x = torch.empty((m.nnode, 3), dtype=torch.float64, device=dev)
+    s = torch.empty((m.nelem, 6), dtype=torch.float64, device=dev)
+    g.download_device(coord=x.data_ptr(), stress=s.data_ptr())
+    hf = h.download(["coord", "stress"])
+    assert np.array_equal(x.cpu().numpy(), hf["coord"]) and np.array_equal(s.cpu().numpy(), hf["stress"])
+    cbt = torch.tensor(cb, dtype=torch.float64, device=dev)
+    vb = torch.empty(m.nelem, dtype=torch.float64, device=dev)
+    hb = torch.empty(m.nelem, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    g.adj_backward_device(cbt.data_ptr(), vb.data_ptr(), hb.data_ptr())
+    vh, hh = h.adj_backward(cb)
+    assert np.array_equal(vb.cpu().numpy(), vh) and np.array_equal(hb.cpu().numpy(), hh)

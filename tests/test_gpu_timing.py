@@ -60,7 +60,7 @@ def test_autotuner_keeps_the_fastest_variant():
         g.set_tuning(4)
 
 
-VARIANTS = {"pass_a": 2, "pass_c": 4, "pass_d": 3, "pass_e": 2, "pass_f": 3}
+VARIANTS = {"pass_a": 2, "pass_c": 4, "pass_d": 5, "pass_e": 2, "pass_f": 5, "pass_b": 3}
 
 
 @pytest.mark.parametrize("name,prec", [("evp3", abi.DES_FP64), ("evp3", abi.DES_MIXED), ("mat2", abi.DES_FP64),
@@ -71,13 +71,13 @@ def test_every_variant_of_every_pass_gives_identical_results(name, prec):
     from dynearthsol_amd.lib import DesModel, TUNE_PASSES
     m = SC.build(name)
     ref = DesModel(m)
-    ref.set_tuning_all(prec, [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0])
+    ref.set_tuning_all(prec, [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0, 0])
     ref.run(20, prec)
     a = ref.download()
     for p in TUNE_PASSES:
         nv = 1 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
         for v in range(nv):
-            choice = [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0]
+            choice = [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0, 0]
             choice[TUNE_PASSES.index(p)] = v
             g = DesModel(m)
             g.set_tuning_all(prec, choice)
@@ -98,7 +98,7 @@ def test_autotuner_tunes_every_pass(prec):
     g, h = DesModel(m), DesModel(m)
     assert all(v == -1 for v, _ in g.tuning_all(prec).values())
     g.run(20, prec)
-    h.set_tuning_all(prec, [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0])
+    h.set_tuning_all(prec, [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0, 0])
     h.run(20, prec)
     t = g.tuning_all(prec)
     print(prec, {p: (v, [round(x, 4) for x in ms]) for p, (v, ms) in t.items()}, flush=True)
@@ -110,7 +110,7 @@ def test_autotuner_tunes_every_pass(prec):
     for k in a:
         assert np.array_equal(a[k], b[k]), k
     with pytest.raises(Exception):
-        g.set_tuning_all(prec, [2, 0, 0, 0, 0])
+        g.set_tuning_all(prec, [2, 0, 0, 0, 0, 0])
 
 
 def test_batched_groups_tune_every_pass():
