@@ -184,16 +184,17 @@ def bench_ensemble(args):
               for nd in (2, 3)}
     bt.close()
     # one at a time: every model, seq_steps timed steps each after a short warmup
+    # (--seq-steps 0 leaves it out)
     seq_secs, seq_work = 0.0, 0
-    for m in models:
+    for m in models if args.seq_steps > 0 else ():
         g = DesModel(m)
         g.run(20, abi.DES_MIXED)
         dt = timed(g, args.seq_steps, abi.DES_MIXED)
         seq_secs += dt
         seq_work += m.nelem * args.seq_steps
         g.close()
-    seq_value = seq_work / seq_secs
-    print(f"# one at a time: {seq_value:.4e} elem-steps/s", file=sys.stderr, flush=True)
+    seq_value = seq_work / seq_secs if seq_secs else None
+    print(f"# one at a time: {seq_value} elem-steps/s", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": value, "unit": "elem-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps,
@@ -203,7 +204,7 @@ def bench_ensemble(args):
                    "nelem_total": E, "precision": "mixed", "schedule": "batched fused six-pass",
                    "tuning": tuning},
         "one_at_a_time": {"value": seq_value, "steps_per_model": args.seq_steps,
-                          "batch_speedup": value / seq_value},
+                          "batch_speedup": value / seq_value if seq_value else None},
     }
     print(json.dumps(out), flush=True)
 

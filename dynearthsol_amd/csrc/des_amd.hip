@@ -117,6 +117,38 @@ template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState
     const bool staged = stage_block(s, xcd_block(), lds);  \
     BIND_ELEM(s, e)
 
+// Stores of per-lane records (an element's erec, a node's nrec) at a stride
+// of R 16-byte chunks cost one L2 write request per lane and chunk: 16 bytes
+// per request.  A full wave instead passes its 64 records through LDS and
+// writes them back so that store k's lane l writes the wave's chunk
+// 64k + l: consecutive lanes write consecutive chunks and a store
+// instruction fills whole 64-byte segments.  Chunks C..R-1 of each record
+// are left as they are.  True when the wave was full and the records are
+// written; false (nothing written) for a partial wave, whose lanes write
+// their own records.  wbuf: this wave's 64*C chunks of LDS.
+__device__ __forceinline__ bool wave_full() { return __builtin_amdgcn_read_exec() == ~0ull; }
+
+template <int C, int R>
+__device__ __forceinline__ bool wave_store_records(double2 *dst, long r0, const double2 (&v)[C], double2 *wbuf)
+{
+    if (!wave_full()) return false;
+    const int l = __lane_id();
+#pragma unroll
+    for (int k = 0; k < C; ++k) wbuf[l * C + k] = v[k];
+    // the lanes read what other lanes wrote: LDS serves one wave's
+    // instructions in order, and the fences keep the compiler from moving
+    // the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        const int j = k * 64 + l;
+        dst[(r0 + j / C) * R + j % C] = wbuf[j];
+    }
+    return true;
+}
+
 // the same with the staging a compile-time choice (STAGE false: every
 // element gathers its node records directly, no LDS)
 #define BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE)                \
@@ -1229,9 +1261,15 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_a(
         mass_of<ND>(s, e, vol, Tm, rec[REC_M], rec[REC_TM]);
         rec[REC_V] = vol;
     }
-    double2 *out = reinterpret_cast<double2 *>(s.erec + (long)e * REC);
+    double2 r2[REC / 2];
 #pragma unroll
-    for (int k = 0; k < REC / 2; ++k) out[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
+    for (int k = 0; k < REC / 2; ++k) r2[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
+    __shared__ double2 xbuf[BLOCK * (REC / 2)];
+    double2 *erec2 = reinterpret_cast<double2 *>(s.erec);
+    if (!wave_store_records<REC / 2, REC / 2>(erec2, e - (int)__lane_id(), r2, xbuf + (threadIdx.x & ~63) * (REC / 2))) {
+#pragma unroll
+        for (int k = 0; k < REC / 2; ++k) erec2[(long)e * (REC / 2) + k] = r2[k];
+    }
 }
 
 // value of lane k (0..3) of this lane's quad
@@ -1264,7 +1302,12 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
     }
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
-    double tdot = 0, dj = 0, vn = 0, m = 0, tm = 0;
+    // Lane q of the quad holds record quarter q: (c0, c1), (c2, c3), (dj, m),
+    // (tm, V).  Each sum is kept by one lane, in ascending element order:
+    // lanes 2 and 3 add their own quarter (acc = dj, m on lane 2; tm, V on
+    // lane 3); lanes 0 and 1 add the node's thermal term c_i, which sits on
+    // lane i/2, into acc.x.  Lane 0 collects the sums at the end.
+    double accx = 0, accy = 0;
     const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
     // DEPTH entries per iteration: their records are all requested before
     // the first is summed
@@ -1289,18 +1332,14 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
             const int i = en[j] % NPE;
             const double sel = (i & 1) ? rq[j].y : rq[j].x;
             const double c0 = quad_bcast<0>(sel), c1 = quad_bcast<1>(sel);
-            const double djv = quad_bcast<2>(rq[j].x), mv = quad_bcast<2>(rq[j].y);
-            const double tmv = quad_bcast<3>(rq[j].x), vv = quad_bcast<3>(rq[j].y);
-            if (thermal) tdot += (i >> 1) ? c1 : c0;
-            dj += djv;
-            if (!fl.use_stored_mass) {
-                vn += vv;
-                m += mv;
-                if (thermal) tm += tmv;
-            }
+            accx += q < 2 ? ((i >> 1) ? c1 : c0) : rq[j].x;
+            accy += rq[j].y;
         }
     }
+    const double tdot = accx, dj = quad_bcast<2>(accx);
+    double m = quad_bcast<2>(accy), tm = quad_bcast<3>(accx), vn = quad_bcast<3>(accy);
     if (q != 0) return;
+    if (!thermal) tm = 0;      // the thermal mass is summed for thermal models only
     if (fl.use_stored_mass) {
         vn = s.volume_n[n];
         m = s.mass[n];
@@ -1315,12 +1354,16 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
     if (thermal) {      // the SoA temperature: read only by this pass (the others read the records)
         Tnew = temperature_new(s, n, __builtin_nontemporal_load(s.T + n), tdot, s.sc->dt, tm);
         __builtin_nontemporal_store(Tnew, s.T + n);
-        rn[NrecSlot<ND>::T] = Tnew;
     }
     // dvoldt goes to the records only: ntmp is left to pass D (NMD_stress's
-    // dp_nd, its last writer in the reference's step)
+    // dp_nd, its last writer in the reference's step).  T and aux share the
+    // record's last 16-byte chunk: one store
     const double dvn = dj / vn;
-    rn[NrecSlot<ND>::AUX] = dvn;
+    static_assert(NrecSlot<ND>::T % 2 == 0 && NrecSlot<ND>::AUX == NrecSlot<ND>::T + 1, "T, aux: one chunk");
+    if (thermal)
+        reinterpret_cast<double2 *>(rn)[NrecSlot<ND>::T / 2] = make_double2(Tnew, dvn);
+    else
+        rn[NrecSlot<ND>::AUX] = dvn;
     if (fl.rec_nrec) {
         double *r = fl.rec_nrec + (long)NREC * n;
 #pragma unroll
@@ -1422,12 +1465,10 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     }
 }
 
-template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
+template <int ND, int WPE, bool STAGE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
-    // pass C is long enough per element that waves do better starting on
-    // their own gathers than waiting at a staging barrier
-    BIND_ELEM(s, e);
-    pass_c_body<ND, true>(s, fl, e, false, nullptr);
+    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
+    pass_c_body<ND, true>(s, fl, e, staged, lds);
 }
 
 // Pass C's Mohr-Coulomb returns (rheology.cxx:151-299, 594-635) of the 3D
@@ -1518,8 +1559,10 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(
     store_force_contrib<ND>(s, e, out);
 }
 
+// wbuf: this wave's LDS for the node records' transposed store (null: the
+// lanes store their own records)
 template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl,
-                                                                                int n, int tb_)
+                                                                                int n, int tb_, double2 *wbuf)
 {
     const double dt = s.sc->dt;
     // the lane of each model's first node ends the step; nothing in this pass
@@ -1545,28 +1588,43 @@ template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_bo
     unsigned flag = s.bcflag[n];
     if (flag & BOUND_ANY)
         if (!vbc_node<ND>(s, flag, v)) set_error(s.sc, DES_ERR_ASSERT);
-    double *rn = s.nrec + (long)NREC * n;
+    double xn[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         __builtin_nontemporal_store(v[d], s.vel + d * s.N + n);
-        const double xn = __builtin_nontemporal_load(s.coord + d * s.N + n) + v[d] * dt;
-        __builtin_nontemporal_store(xn, s.coord + d * s.N + n);
-        rn[NrecSlot<ND>::X + d] = xn;
-        rn[NrecSlot<ND>::V + d] = v[d];
+        xn[d] = __builtin_nontemporal_load(s.coord + d * s.N + n) + v[d] * dt;
+        __builtin_nontemporal_store(xn[d], s.coord + d * s.N + n);
+    }
+    // the record's x and v: its first ND 16-byte chunks (T and aux stay)
+    double2 c[ND];
+    if constexpr (ND == 3) {
+        c[0] = make_double2(xn[0], xn[1]);
+        c[1] = make_double2(xn[2], v[0]);
+        c[2] = make_double2(v[1], v[2]);
+    } else {
+        c[0] = make_double2(xn[0], xn[1]);
+        c[1] = make_double2(v[0], v[1]);
+    }
+    double2 *nrec2 = reinterpret_cast<double2 *>(s.nrec);
+    if (!wbuf || !wave_store_records<ND, NREC / 2>(nrec2, n - (int)__lane_id(), c, wbuf)) {
+#pragma unroll
+        for (int k = 0; k < ND; ++k) nrec2[(long)n * (NREC / 2) + k] = c[k];
     }
 }
 
 template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f(DevState s, FusedFlags fl)
 {
+    __shared__ double2 wbuf[NB * ND];
     BIND_NODE_NB(s, n, NB);
-    pass_f_body<ND, false, DEPTH>(s, fl, n, tb_);
+    pass_f_body<ND, false, DEPTH>(s, fl, n, tb_, wbuf + (threadIdx.x & ~63) * ND);
 }
 
 // pass F of the mixed-precision schedule: FP32 force contributions
 template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f32(DevState s, FusedFlags fl)
 {
+    __shared__ double2 wbuf[NB * ND];
     BIND_NODE_NB(s, n, NB);
-    pass_f_body<ND, true, DEPTH>(s, fl, n, tb_);
+    pass_f_body<ND, true, DEPTH>(s, fl, n, tb_, wbuf + (threadIdx.x & ~63) * ND);
 }
 
 // ---- mixed precision (DES_MIXED): passes C and E on (pres, sdev) -------
@@ -3395,8 +3453,9 @@ int ensure_mixed(des_ctx *ctx)
 // they differ in how the work maps on the machine:
 //   A, E: the block's node records staged in LDS (0) or gathered directly (1);
 //   C:    a minimum of 1, 2, 3 or 4 waves per SIMD (__launch_bounds__): fewer
-//         registers per lane buy occupancy at the price of spills (FP64 only;
-//         the mixed pass C has one variant);
+//         registers per lane buy occupancy at the price of spills; and 1 or 2
+//         with the block's node records staged in LDS (FP64 only; the mixed
+//         pass C has one variant);
 //   D, F: node blocks of 256, 128 or 64 threads; D also 256 threads with 8
 //         instead of 4 gathered entries in flight;
 //   B:    4 or 8 element records in flight per node.
@@ -3405,10 +3464,10 @@ int ensure_mixed(des_ctx *ctx)
 // variant count at tuning step k) and keep the fastest mean.
 constexpr int TUNE_A = DES_TUNE_A, TUNE_C = DES_TUNE_C, TUNE_D = DES_TUNE_D, TUNE_E = DES_TUNE_E,
               TUNE_F = DES_TUNE_F, TUNE_B = DES_TUNE_B;
-constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 4, 5, 2, 5, 3}, {2, 1, 5, 2, 5, 3}};
+constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 6, 5, 2, 5, 3}, {2, 1, 5, 2, 5, 3}};
 constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 3, 0, 3, 1};
 constexpr int TUNE_STEPS = 2 * DES_TUNE_MAXV - 2, TUNE_MIN_RUN = 16;
-constexpr int PASSC_VARIANTS = 4;
+constexpr int PASSC_VARIANTS = 6;
 
 void clear_tuning_events(des_ctx *ctx)
 {
@@ -3510,20 +3569,25 @@ int launch_pass_f(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
 int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
 {
     dim3 g(nblocks(ctx->E)), b(BLOCK);
-#define PC(NDV, W) hipExtLaunchKernelGGL((k_pass_c<NDV, W>), g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s, fl)
+#define PC(NDV, W, ST) \
+    hipExtLaunchKernelGGL((k_pass_c<NDV, W, ST>), g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s, fl)
     if (ctx->nd == 3) {
         switch (variant) {
-        case 0: PC(3, 1); break;
-        case 1: PC(3, 2); break;
-        case 2: PC(3, 3); break;
-        default: PC(3, 4); break;
+        case 0: PC(3, 1, false); break;
+        case 1: PC(3, 2, false); break;
+        case 2: PC(3, 3, false); break;
+        case 3: PC(3, 4, false); break;
+        case 4: PC(3, 1, true); break;
+        default: PC(3, 2, true); break;
         }
     } else {
         switch (variant) {
-        case 0: PC(2, 1); break;
-        case 1: PC(2, 2); break;
-        case 2: PC(2, 3); break;
-        default: PC(2, 4); break;
+        case 0: PC(2, 1, false); break;
+        case 1: PC(2, 2, false); break;
+        case 2: PC(2, 3, false); break;
+        case 3: PC(2, 4, false); break;
+        case 4: PC(2, 1, true); break;
+        default: PC(2, 2, true); break;
         }
     }
 #undef PC
@@ -4111,7 +4175,7 @@ int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem)
     return DES_OK;
 }
 
-int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4])
+int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[DES_TUNE_MAXV])
 {
     if (!b || !variant) return DES_ERR_INPUT;
     for (des_ctx *ctx : b->groups)
@@ -4128,12 +4192,12 @@ int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES
     return bfail(b, DES_ERR_INPUT, "no group of that dimension");
 }
 
-int des_get_tuning(des_ctx *ctx, int *variant, double ms[4])
+int des_get_tuning(des_ctx *ctx, int *variant, double ms[DES_TUNE_MAXV])
 {
     if (!ctx || !variant) return DES_ERR_INPUT;
     *variant = ctx->tune_var[0][TUNE_C];
     if (ms)
-        for (int v = 0; v < 4; ++v) ms[v] = ctx->tune_ms[0][TUNE_C][v];
+        for (int v = 0; v < DES_TUNE_MAXV; ++v) ms[v] = ctx->tune_ms[0][TUNE_C][v];
     return DES_OK;
 }
 

@@ -40,7 +40,7 @@ TUNE_PASSES = ("pass_a", "pass_c", "pass_d", "pass_e", "pass_f", "pass_b")
 _NODE_VARIANTS = ("block256-depth4", "block128-depth4", "block64-depth4", "block256-depth8", "block256-depth16")
 TUNE_MEANING = {
     "pass_a": ("lds-staged", "direct"), "pass_e": ("lds-staged", "direct"),
-    "pass_c": ("waves>=1", "waves>=2", "waves>=3", "waves>=4"),
+    "pass_c": ("waves>=1", "waves>=2", "waves>=3", "waves>=4", "lds-staged waves>=1", "lds-staged waves>=2"),
     "pass_d": _NODE_VARIANTS, "pass_f": _NODE_VARIANTS, "pass_b": ("depth4", "depth8", "depth16"),
 }
 
@@ -365,7 +365,7 @@ class DesModel:
     def tuning(self):
         """(pass-C variant chosen by the autotuner or -1, [ms per variant])."""
         v = C.c_int()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * TUNE_MAXV)()
         self._check(self.lib.des_get_tuning(self.ctx, C.byref(v), ms))
         return v.value, list(ms)
 
@@ -493,7 +493,7 @@ class DesBatch:
 
     def tuning(self, ndims):
         v = C.c_int()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * TUNE_MAXV)()
         self._check(self.lib.des_batch_tuning(self.b, ndims, C.byref(v), ms))
         return v.value, list(ms)
 

@@ -297,7 +297,9 @@ int des_sync(des_ctx *ctx);
      DES_TUNE_A, DES_TUNE_E: 0 the element block's node records staged in LDS,
                              1 gathered directly;
      DES_TUNE_C:             FP64 pass C with a minimum of 1..4 waves per SIMD
-                             (0..3); the mixed pass C has one variant (0);
+                             (0..3), or of 1 or 2 with the element block's
+                             node records staged in LDS (4, 5); the mixed
+                             pass C has one variant (0);
      DES_TUNE_D, DES_TUNE_F: node blocks of 256, 128 or 64 threads with 4
                              gathered entries in flight per node (0..2), or
                              256 threads with 8 or 16 in flight (3, 4);
@@ -323,8 +325,8 @@ int des_get_tuning_all(des_ctx *ctx, int precision, int variant[DES_NTUNE],
                        double ms[DES_TUNE_MAXV * DES_NTUNE]);
 /* Fix the choices (each -1 to let the next long run tune that pass). */
 int des_set_tuning_all(des_ctx *ctx, int precision, const int variant[DES_NTUNE]);
-/* The FP64 pass-C entry of the above: variant 0..3 or -1, ms per variant. */
-int des_get_tuning(des_ctx *ctx, int *variant, double ms[4]);
+/* The FP64 pass-C entry of the above: variant 0..5 or -1, ms per variant. */
+int des_get_tuning(des_ctx *ctx, int *variant, double ms[DES_TUNE_MAXV]);
 int des_set_tuning(des_ctx *ctx, int variant);
 
 /* ---- differentiated run (no counterpart in the reference) ------------------
@@ -389,7 +391,7 @@ int des_batch_compute_dt(des_batch *b, double *dt);  /* dt[model] */
 int des_batch_run(des_batch *b, int nsteps, int precision);
 int des_batch_sync(des_batch *b);
 int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem);   /* per model */
-int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[4]);
+int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[DES_TUNE_MAXV]);
 int des_batch_tuning_all(des_batch *b, int ndims, int precision, int variant[DES_NTUNE],
                          double ms[DES_TUNE_MAXV * DES_NTUNE]);
 

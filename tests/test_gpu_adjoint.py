@@ -83,36 +83,78 @@ def test_gradient_matches_finite_differences():
 
 def test_gradient_matches_finite_differences_on_random_elements():
     """64 elements drawn at random (seed 5), as the adjoint configuration's
-    check prescribes: central differences (h = 1e-2 in log viscosity) of the
-    HIP forward, maxwell, no damping, 20 steps, loss on the displacement.
-    Most random elements move the loss by little more than the FP64
-    forward's own rounding, so the bound is 1e-3 relative plus 1e-6 of the
-    largest gradient; the 1e-5 relative check is on the largest gradients
-    (test_gradient_matches_finite_differences)."""
+    check prescribes, each to 1e-5 relative, over the configuration's
+    horizon: 2000 steps, checkpointed every 100.
+
+    A finite difference resolves a gradient only above the FP64 forward's
+    rounding noise, about 1e-9 in d loss / d log(visc) here.  With the loss
+    on the top surface alone some elements have gradients of 1e-6 of the
+    largest, at that noise (measured: 2.4e-4 relative on one of the 64 after
+    1000 steps).  So the check weighs the vertical displacement of every
+    node (seeded weights): the same reverse sweep with a coord_bar that is
+    non-zero everywhere, which gives every element a gradient of 2e-3 of the
+    largest or more.  The difference: central differences at h = 0.4, 0.2,
+    0.1 in log viscosity combined by two levels of Richardson extrapolation
+    (error O(h^6)).  Maxwell, no damping.  Measured worst: 1.3e-6."""
     from dynearthsol_amd.lib import DesModel
-    m, cb = _case("maxwell", nelem=2000, damping=0)
-    g, vb, _ = _gpu_grad(m, cb, 20, 8)
+    nsteps = 2000
+    m, _ = _case("maxwell", nelem=2000, damping=0)
+    cb = np.zeros((m.nnode, 3))
+    cb[:, 2] = np.random.default_rng(3).standard_normal(m.nnode)
+    g, vb, _ = _gpu_grad(m, cb, nsteps, 100)
+    g.close()
     x0 = m.coord.copy()
-    h = 1e-2
-    scale = np.max(np.abs(vb * m.elem_visc))
     picks = np.random.default_rng(5).choice(m.nelem, 64, replace=False)
-    worst = 0.0
+    ad_all = vb * m.elem_visc
+    gg = DesModel(m)
+
+    def loss(e, fac):
+        a = m.elem_visc.copy()
+        a[e] *= fac
+        gg.reset()
+        gg.set_elem_params(elem_visc=a)
+        gg.run(nsteps)
+        return float(np.sum(cb * (gg.download(["coord"])["coord"] - x0)))
+
+    hs = (0.4, 0.2, 0.1)
+    errs = []
     for e in picks:
-        vals = []
-        for sgn in (1, -1):
-            a = m.elem_visc.copy()
-            a[e] *= math.exp(sgn * h)
-            gg = DesModel(m)
-            gg.set_elem_params(elem_visc=a)
-            gg.run(20)
-            vals.append(float(np.sum(cb * (gg.download(["coord"])["coord"] - x0))))
-            gg.close()
-        fd = (vals[0] - vals[1]) / (2 * h)          # d loss / d log(visc)
-        ad = vb[e] * m.elem_visc[e]
-        bound = 1e-3 * abs(fd) + 1e-6 * scale
-        worst = max(worst, abs(ad - fd) / bound)
-        assert abs(ad - fd) <= bound, (e, ad, fd)
-    print(f"64 random elements: worst difference {worst:.2f} of the bound")
+        D = [(loss(e, math.exp(h)) - loss(e, math.exp(-h))) / (2 * h) for h in hs]
+        R1 = [(4 * D[k + 1] - D[k]) / 3 for k in range(2)]
+        fd = (16 * R1[1] - R1[0]) / 15              # d loss / d log(visc)
+        ad = ad_all[e]
+        errs.append(abs(ad - fd) / abs(fd))
+        print(f"e {e:5d} |grad|/max {abs(ad) / np.abs(ad_all).max():.1e} adjoint {ad: .10e} fd {fd: .10e} "
+              f"rel {errs[-1]:.1e}", flush=True)
+    print(f"64 random elements: worst relative difference {max(errs):.2e}")
+    assert max(errs) <= 1e-5
+
+
+def test_plastic_gradient_matches_finite_differences():
+    """Elasto-visco-plastic with a deviatoric initial stress and low cohesion,
+    so that elements yield from the first step and stay on the yield
+    surface: viscosity and cohesion gradients of the four elements with the
+    largest of each, against central differences (h = 1e-3, log space) of
+    the HIP forward, 1e-5 relative."""
+    from dynearthsol_amd.lib import DesModel
+    m, cb = _case("plastic", perturb=30e6, coh_scale=0.2, damping=0)
+    g, vb, hb = _gpu_grad(m, cb, 10, 4)
+    assert np.count_nonzero(hb) > 10
+    gg = DesModel(m)
+    h = 1e-3
+    for which, grad, base in (("elem_visc", vb, m.elem_visc), ("elem_cohesion", hb, m.elem_cohesion)):
+        for e in np.argsort(-np.abs(grad * base))[:4]:
+            vals = []
+            for sgn in (1, -1):
+                a = base.copy()
+                a[e] *= math.exp(sgn * h)
+                gg.reset()
+                gg.set_elem_params(**{which: a})
+                gg.run(10)
+                vals.append(float(np.sum(cb * gg.download(["coord"])["coord"])))
+            fd = (vals[0] - vals[1]) / (2 * h) / base[e]
+            print(f"{which} e {e}: adjoint {grad[e]:.8e} fd {fd:.8e}", flush=True)
+            assert abs(grad[e] - fd) <= 1e-5 * abs(fd), (which, e, grad[e], fd)
 
 
 def test_window_size_does_not_change_the_gradient():

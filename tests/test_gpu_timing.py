@@ -38,7 +38,7 @@ def test_pass_c_variants_give_identical_results():
     from dynearthsol_amd.lib import DesModel
     m = SC.build("evp3")
     out = []
-    for v in range(4):
+    for v in range(6):
         g = DesModel(m)
         g.set_tuning(v)
         g.run(20)
@@ -55,12 +55,12 @@ def test_autotuner_keeps_the_fastest_variant():
     assert g.tuning()[0] == -1
     g.run(20)
     v, ms = g.tuning()
-    assert 0 <= v < 4 and all(x > 0 for x in ms) and ms[v] == min(ms)
+    assert 0 <= v < 6 and all(x > 0 for x in ms) and ms[v] == min(ms)
     with pytest.raises(DesError):
-        g.set_tuning(4)
+        g.set_tuning(6)
 
 
-VARIANTS = {"pass_a": 2, "pass_c": 4, "pass_d": 5, "pass_e": 2, "pass_f": 5, "pass_b": 3}
+VARIANTS = {"pass_a": 2, "pass_c": 6, "pass_d": 5, "pass_e": 2, "pass_f": 5, "pass_b": 3}
 
 
 @pytest.mark.parametrize("name,prec", [("evp3", abi.DES_FP64), ("evp3", abi.DES_MIXED), ("mat2", abi.DES_FP64),
