@@ -277,11 +277,15 @@ def main():
     ap.add_argument("--adjoint-nelem", type=int, default=200_000)
     ap.add_argument("--ensemble-scale", type=float, default=1.0)
     ap.add_argument("--seq-steps", type=int, default=100)
+    ap.add_argument("--sweep-order", type=int, choices=(0, 1), default=1,
+                    help="1: the node passes walk each XCD's blocks downwards (default); 0: upwards")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
               "not part of this build", file=sys.stderr)
         sys.exit(2)
+    from dynearthsol_amd import lib as L
+    L.set_sweep_order(args.sweep_order)
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "adjoint":
@@ -423,6 +427,7 @@ def main():
         "dtype": "f64" if primary == abi.DES_FP64 else "f32 deviator/rheology + f64 pressure/geometry/force accumulation",
         "data": "synthetic",
         "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
+                   "sweep_order": args.sweep_order,
                    "precision": "fp64" if primary == abi.DES_FP64 else "mixed",
                    "tuning": {("fp64" if p == abi.DES_FP64 else "mixed"):
                               {k: {"variant": TUNE_MEANING[k][v], "ms": [round(x, 5) for x in ms if x > 0]}

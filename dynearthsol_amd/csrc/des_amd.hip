@@ -40,21 +40,27 @@ inline int nblocks(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 // ---------------------------------------------------------------- kernels
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Renumbering them so that XCD x runs one contiguous range of logical
-// blocks gives every XCD the same contiguous slab of the (renumbered) mesh in
-// every pass: what one pass writes is re-read from the same L2 by the next.
-__device__ __forceinline__ int xcd_block()
+// L2, in ascending workgroup order.  Renumbering them so that XCD x runs one
+// contiguous range of logical blocks gives every XCD the same contiguous slab
+// of the (renumbered) mesh in every pass: what one pass writes is re-read
+// from the same L2 by the next.  With rev set, each XCD walks its range from
+// the top down: a pass that follows one walking upwards starts where that one
+// ended, on the lines it wrote last, which are still in the L2 and the 256 MB
+// Infinity Cache when the pass's data are larger than both.
+__device__ __forceinline__ int xcd_block(int rev = 0)
 {
     const int nb = gridDim.x, b = blockIdx.x;
     const int q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + min(x, r) + (b >> 3);
+    int j = b >> 3;
+    if (rev) j = q + (x < r ? 1 : 0) - 1 - j;    // XCD x has q (+1 if x < r) blocks
+    return x * q + min(x, r) + j;
 }
 
 // Element / node index of this thread.  For batched models the block's model
 // supplies the parameters and scalars (the kernel's by-value DevState is
 // rebound), and padding lanes past the model's last element return.
 #define BIND_ELEM(s, e)                                \
-    const int blk_ = xcd_block();                      \
+    const int blk_ = xcd_block(s.rev);                 \
     const int e = blk_ * BLOCK + threadIdx.x;          \
     if (s.eblk_model) {                                \
         if (e >= s.eblk_lim[blk_]) return;             \
@@ -66,7 +72,7 @@ __device__ __forceinline__ int xcd_block()
     }
 
 #define BIND_NODE(s, n)                                \
-    const int blk_ = xcd_block();                      \
+    const int blk_ = xcd_block(s.rev);                 \
     const int n = blk_ * BLOCK + threadIdx.x;          \
     if (s.nblk_model) {                                \
         if (n >= s.nblk_lim[blk_]) return;             \
@@ -114,7 +120,7 @@ template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState
 
 #define BIND_ELEM_STAGED(s, e, lds, staged)                \
     __shared__ double2 lds[STAGE_MAX * (NREC / 2)];        \
-    const bool staged = stage_block(s, xcd_block(), lds);  \
+    const bool staged = stage_block(s, xcd_block(s.rev), lds); \
     BIND_ELEM(s, e)
 
 // Stores of per-lane records (an element's erec, a node's nrec) at a stride
@@ -153,13 +159,13 @@ __device__ __forceinline__ bool wave_store_records(double2 *dst, long r0, const 
 // element gathers its node records directly, no LDS)
 #define BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE)                \
     __shared__ double2 lds[(STAGE) ? STAGE_MAX * (NREC / 2) : 1];   \
-    const bool staged = (STAGE) && stage_block(s, xcd_block(), lds); \
+    const bool staged = (STAGE) && stage_block(s, xcd_block(s.rev), lds); \
     BIND_ELEM(s, e)
 
 // Node lane n of a kernel launched in blocks of NB threads (NB divides
 // BLOCK); a batched group's block tables are kept per BLOCK nodes, entry tb_.
 #define BIND_NODE_NB(s, n, NB)                         \
-    const int n = xcd_block() * (NB) + threadIdx.x;    \
+    const int n = xcd_block(s.rev) * (NB) + threadIdx.x; \
     const int tb_ = n / BLOCK;                         \
     if (s.nblk_model) {                                \
         if (n >= s.nblk_lim[tb_]) return;              \
@@ -1291,7 +1297,7 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
     // record with one 16-byte load per lane and shares it through DPP; every
     // lane of the quad sums the same values in the same (ascending element)
     // order, and lane 0 finishes the node
-    const int blk_ = xcd_block();
+    const int blk_ = xcd_block(s.rev);
     const int t = blk_ * BLOCK + threadIdx.x, n = t >> 2, q = t & 3;
     if (s.nblk_model) {
         const int nb = blk_ >> 2;
@@ -1422,7 +1428,8 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     load_tensor<ND>(s.stress, E, e, st);
     if (fl.rec_stress) store_tensor<ND>(fl.rec_stress, E, e, st);
     if (rot) jaumann<ND>(st, dtp, w);
-    double syy = s.stressyy[e], dp;
+    // stressyy exists only under plane strain (2D); nothing else reads or writes it
+    double syy = (ND == 2 && s.P->is_plane_strain) ? s.stressyy[e] : 0.0, dp;
     double pl = __builtin_nontemporal_load(s.plstrain + e), dpl = __builtin_nontemporal_load(s.delta_plstrain + e);
     if (fl.rec_plstrain) fl.rec_plstrain[e] = pl;
     if (fl.rec_volume_old) fl.rec_volume_old[e] = s.volume_old[e];
@@ -2174,6 +2181,7 @@ namespace {
 // ascending host order, so every sum runs in the reference's order and the
 // results are bitwise those of the host numbering.
 int g_mesh_order = 1;    // 1: Hilbert-curve order, 0: host numbering (des_set_mesh_order)
+int g_sweep = 1;         // 1: node passes walk their blocks downwards (des_set_sweep_order)
 
 // Position along the Hilbert curve of the cell X[0..nd) (bits per axis):
 // J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 381
@@ -2704,6 +2712,7 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
 }
 
 void des_set_mesh_order(int mode) { g_mesh_order = (mode >= 0 && mode <= 3) ? mode : 1; }
+void des_set_sweep_order(int mode) { g_sweep = mode ? 1 : 0; }
 
 void des_destroy(des_ctx *ctx)
 {
@@ -3677,11 +3686,18 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         }
         return DES_OK;
     };
+    // the element passes walk their blocks upwards, the node passes (B, D, F)
+    // downwards: each pass starts on what the one before it wrote last
+    const int down = g_sweep ? 1 : 0;
     ARM(false);
+    ctx->s.rev = 0;
     if ((rc = tunable(TUNE_A, [&](int v) { return launch_pass_a(ctx, fl, v); }))) return rc;
     MARK(0);
     ARM(false);
-    if ((rc = tunable(TUNE_B, [&](int v) { return launch_pass_b(ctx, fl, v); }))) return rc;
+    ctx->s.rev = down;
+    rc = tunable(TUNE_B, [&](int v) { return launch_pass_b(ctx, fl, v); });
+    ctx->s.rev = 0;
+    if (rc) return rc;
     MARK(1);
     ARM(false);
     rc = tunable(TUNE_C, [&](int v) -> int {
@@ -3699,7 +3715,10 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
         MARK(8);
     }
     ARM(false);
-    if ((rc = tunable(TUNE_D, [&](int v) { return launch_pass_d(ctx, fl, v); }))) return rc;
+    ctx->s.rev = down;
+    rc = tunable(TUNE_D, [&](int v) { return launch_pass_d(ctx, fl, v); });
+    ctx->s.rev = 0;
+    if (rc) return rc;
     MARK(3);
     ARM(false);
     if ((rc = tunable(TUNE_E, [&](int v) { return launch_pass_e(ctx, fl, mixed, v); }))) return rc;
@@ -3712,7 +3731,10 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     ARM(false);
     FusedFlags ff = fl;
     ff.step_end = 1;
-    if ((rc = tunable(TUNE_F, [&](int v) { return launch_pass_f(ctx, ff, mixed, v); }))) return rc;
+    ctx->s.rev = down;
+    rc = tunable(TUNE_F, [&](int v) { return launch_pass_f(ctx, ff, mixed, v); });
+    ctx->s.rev = 0;
+    if (rc) return rc;
     MARK(6);
     if (tuning && k == TUNE_STEPS - 1 && !ctx->tune.empty()) {
         // the fastest mean per untuned pass

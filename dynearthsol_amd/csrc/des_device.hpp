@@ -99,6 +99,9 @@ struct DevState {
     const int *mblk_off;                          // first element block of each model (+ total)
     const int *e0dev;                             // per model: device index of host element 0
     int nmodels;
+    // 1: each XCD walks its contiguous range of blocks from the top down
+    // (the fused schedule's node passes; see xcd_block)
+    int rev;
     // elements whose Mohr-Coulomb return is left to the return kernel of the
     // fused schedule: count, element ids, and per entry the trial stress and
     // the stress trace before the update (8 doubles)

@@ -31,7 +31,7 @@ EXPORTS = (
     "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
     "des_worst_quality_noalloc", "des_get_tuning_all", "des_set_tuning_all", "des_batch_tuning_all",
     "des_write_checkpoint", "des_remesh_finish", "des_upload_fields_device", "des_download_fields_device",
-    "des_adj_backward_device",
+    "des_adj_backward_device", "des_set_sweep_order",
 )
 
 NTUNE = 6
@@ -119,6 +119,8 @@ def load():
     lib.des_batch_tuning_all.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.des_set_mesh_order.argtypes = [C.c_int]
     lib.des_set_mesh_order.restype = None
+    lib.des_set_sweep_order.argtypes = [C.c_int]
+    lib.des_set_sweep_order.restype = None
     lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
     lib.des_batch_destroy.argtypes = [vp]
     lib.des_batch_destroy.restype = None
@@ -147,6 +149,13 @@ def set_mesh_order(mode):
     """Device numbering of meshes created from now on in this process:
     1 Hilbert-curve order (default), 0 the host's order.  Results do not depend on it."""
     load().des_set_mesh_order(int(mode))
+
+
+def set_sweep_order(mode):
+    """Block order of the fused node passes, process-wide: 1 (default) each
+    XCD walks passes B, D and F from the top of its range down, 0 upwards.
+    Results do not depend on it."""
+    load().des_set_sweep_order(int(mode))
 
 
 class DesModel:

@@ -162,6 +162,12 @@ void des_destroy(des_ctx *ctx);
    sums its elements in ascending host order, so results do not depend on the
    setting. */
 void des_set_mesh_order(int mode);
+/* Block order of the fused schedule's node passes (process-wide; no
+   counterpart in the reference): 1 (default) walks passes B, D and F from the
+   top of each XCD's block range down, so that each pass starts on the data the
+   pass before it wrote last; 0 walks every pass upwards.  Each element and
+   node is computed the same way in either order, so results are the same. */
+void des_set_sweep_order(int mode);
 const char *des_last_error(const des_ctx *ctx);
 
 int des_upload_fields(des_ctx *ctx, const des_fields *f);

@@ -144,3 +144,28 @@ def test_gradient_is_independent_of_numbering():
         g.adj_forward(12, 5)
         out.append(g.adj_backward(cb))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("name", ["evp3", "psevp2", "mat3", "bc3", "dynmat3"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_fused_run_is_independent_of_sweep_order(name, mixed):
+    """The node passes' block order (des_set_sweep_order) changes which
+    workgroup computes a node and when, never how: bitwise equal runs."""
+    from dynearthsol_amd.lib import DesModel, set_sweep_order
+    if mixed and name == "psevp2":
+        pytest.skip("mixed precision does not cover plane strain")
+    m = SC.build(name)
+    prec = abi.DES_MIXED if mixed else abi.DES_FP64
+    print(name, m.nelem, "elements, mixed" if mixed else "elements, fp64", flush=True)
+    out = []
+    for order in (0, 1):
+        set_sweep_order(order)
+        try:
+            g = DesModel(m)
+            g.run(20, prec)
+            out.append((g.download(), g.scalars()))
+            g.close()
+        finally:
+            set_sweep_order(1)
+    _same(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
