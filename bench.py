@@ -182,6 +182,17 @@ def bench_ensemble(args):
     from dynearthsol_amd.lib import TUNE_MEANING
     tuning = {f"{nd}d": {p: TUNE_MEANING[p][v] for p, (v, _) in bt.tuning_all(nd, abi.DES_MIXED).items()}
               for nd in (2, 3)}
+    # the same batch in one persistent cooperative launch per run (both
+    # dimension groups in one kernel, grid barriers between the passes)
+    bt.set_launch_mode(1)
+    psteps = max(1, min(args.steps, 50))
+    bt.run(psteps, abi.DES_MIXED)
+    bt.sync()
+    t0 = time.perf_counter()
+    bt.run(psteps, abi.DES_MIXED)
+    bt.sync()
+    psecs = time.perf_counter() - t0
+    print(f"# batch, persistent launch: {1e3 * psecs / psteps:.4f} ms per step", file=sys.stderr, flush=True)
     bt.close()
     # one at a time: every model, seq_steps timed steps each after a short warmup
     # (--seq-steps 0 leaves it out)
@@ -205,6 +216,8 @@ def bench_ensemble(args):
                    "tuning": tuning},
         "one_at_a_time": {"value": seq_value, "steps_per_model": args.seq_steps,
                           "batch_speedup": value / seq_value if seq_value else None},
+        "persistent_launch": {"value": E * psteps / psecs, "ms_per_step": 1e3 * psecs / psteps, "steps": psteps,
+                              "vs_per_pass_launches": (E * psteps / psecs) / value},
     }
     print(json.dumps(out), flush=True)
 
@@ -394,6 +407,17 @@ def main():
                    "mixed_rel_err_vs_fp64": {k: rel(a[k], b[k]) for k in names}}
         g2.close()
     g.close()
+    # the same model in one persistent cooperative launch per run (grid
+    # barriers between the passes), for comparison with the per-pass launches
+    gp = DesModel(m)
+    gp.set_launch_mode(1)
+    gp.run(10, primary)
+    psteps = max(1, min(args.steps, 50))
+    tp = timed(gp, psteps, primary)
+    gp.close()
+    persistent = {"steps": psteps, "ms_per_step": 1e3 * tp / psteps,
+                  "vs_per_pass_launches": (secs / args.steps) / (tp / psteps)}
+    print(f"# persistent launch: {1e3 * tp / psteps:.4f} ms/step", file=sys.stderr, flush=True)
     tot_ms, tot_n = kernel_totals[kernel]
     proc_avg = tot_ms / tot_n
     achieved = (be * E + bn * N) / (proc_avg * 1e-3) / 1e9
@@ -442,6 +466,7 @@ def main():
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},
         "fp64_vs_mixed": compare,
+        "persistent_launch": persistent,
         "adjoint_over_forward": adjoint["gradient_over_forward"] if adjoint else None,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

@@ -14,6 +14,7 @@
 // bitwise run-to-run determinism.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
+#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <cmath>
@@ -59,8 +60,8 @@ __device__ __forceinline__ int xcd_block(int rev = 0)
 // Element / node index of this thread.  For batched models the block's model
 // supplies the parameters and scalars (the kernel's by-value DevState is
 // rebound), and padding lanes past the model's last element return.
-#define BIND_ELEM(s, e)                                \
-    const int blk_ = xcd_block(s.rev);                 \
+#define BIND_ELEM_AT(s, e, BLK)                        \
+    const int blk_ = (BLK);                            \
     const int e = blk_ * BLOCK + threadIdx.x;          \
     if (s.eblk_model) {                                \
         if (e >= s.eblk_lim[blk_]) return;             \
@@ -70,6 +71,7 @@ __device__ __forceinline__ int xcd_block(int rev = 0)
     } else if (e >= s.E) {                             \
         return;                                        \
     }
+#define BIND_ELEM(s, e) BIND_ELEM_AT(s, e, xcd_block(s.rev))
 
 #define BIND_NODE(s, n)                                \
     const int blk_ = xcd_block(s.rev);                 \
@@ -118,11 +120,6 @@ template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState
     }
 }
 
-#define BIND_ELEM_STAGED(s, e, lds, staged)                \
-    __shared__ double2 lds[STAGE_MAX * (NREC / 2)];        \
-    const bool staged = stage_block(s, xcd_block(s.rev), lds); \
-    BIND_ELEM(s, e)
-
 // Stores of per-lane records (an element's erec, a node's nrec) at a stride
 // of R 16-byte chunks cost one L2 write request per lane and chunk: 16 bytes
 // per request.  A full wave instead passes its 64 records through LDS and
@@ -155,17 +152,11 @@ __device__ __forceinline__ bool wave_store_records(double2 *dst, long r0, const 
     return true;
 }
 
-// the same with the staging a compile-time choice (STAGE false: every
-// element gathers its node records directly, no LDS)
-#define BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE)                \
-    __shared__ double2 lds[(STAGE) ? STAGE_MAX * (NREC / 2) : 1];   \
-    const bool staged = (STAGE) && stage_block(s, xcd_block(s.rev), lds); \
-    BIND_ELEM(s, e)
-
 // Node lane n of a kernel launched in blocks of NB threads (NB divides
 // BLOCK); a batched group's block tables are kept per BLOCK nodes, entry tb_.
-#define BIND_NODE_NB(s, n, NB)                         \
-    const int n = xcd_block(s.rev) * (NB) + threadIdx.x; \
+#define BIND_NODE_NB(s, n, NB) BIND_NODE_NB_AT(s, n, NB, xcd_block(s.rev))
+#define BIND_NODE_NB_AT(s, n, NB, BLK)                 \
+    const int n = (BLK) * (NB) + threadIdx.x;          \
     const int tb_ = n / BLOCK;                         \
     if (s.nblk_model) {                                \
         if (n >= s.nblk_lim[tb_]) return;              \
@@ -577,11 +568,8 @@ __device__ double ref_pressure(const des_params *P, double z)   // matprops.cxx:
 
 // traction of one loaded boundary facet (apply_stress_bcs, bc.cxx:502-556):
 // ftrac[k] = p * normal / NODES_PER_FACET
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevState s, const int *facets,
-                                                                          int nf)
+template <int ND> __device__ __forceinline__ void facet_traction_item(DevState s, const int *facets, int t)
 {
-    int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= nf) return;
     int k = facets[t];
     if (s.fmodel) bind_model<ND>(s, s.fmodel[t]);
     const des_params *P = s.P;
@@ -627,6 +615,13 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevS
     }
 #pragma unroll
     for (int d = 0; d < ND; ++d) s.ftrac[k * ND + d] = p * normal[d] / ND;
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevState s, const int *facets,
+                                                                          int nf)
+{
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t < nf) facet_traction_item<ND>(s, facets, t);
 }
 
 // node part of update_force: gather, stress BCs, elastic foundation,
@@ -987,12 +982,11 @@ template <int ND> __device__ __forceinline__ void elem0_props(const DevState &s,
     rho0 = mat_rho(s, e0, elem_mean<ND>(s.T, c));
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb)
+template <int ND> __device__ __forceinline__ void dt_final_body(const DevState &s, int nb, double (*sm)[BLOCK])
 {
     double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     for (int b = threadIdx.x; b < nb; b += BLOCK)
         for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
-    __shared__ double sm[3][BLOCK];
     block_min3(sm, v);
     if (threadIdx.x == 0) {
         double bulkm0 = 0, rho0 = 1;
@@ -1001,36 +995,56 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s
     }
 }
 
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final(DevState s, int nb)
+{
+    __shared__ double sm[3][BLOCK];
+    dt_final_body<ND>(s, nb, sm);
+}
+
 // compute_dt of batched models: element block b holds elements of one model
 // only, so its minima go to slot b; one block per model then reduces that
 // model's slots and evaluates bulkm and rho of the model's first element.
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial_b(DevState s)
+// the minima of element block blk in red[3 blk ..]: every lane of the
+// workgroup takes part (the block reduction has barriers)
+template <int ND> __device__ __forceinline__ void dt_partial_blk(DevState s, int blk, double (*sm)[BLOCK])
 {
-    const int blk = xcd_block();
     const int e = blk * BLOCK + threadIdx.x;
-    bind_model<ND>(s, s.eblk_model[blk]);
+    bool valid = e < s.E;
+    if (s.eblk_model) {
+        bind_model<ND>(s, s.eblk_model[blk]);
+        valid = e < s.eblk_lim[blk];
+    }
     double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
-    if (e < s.eblk_lim[blk]) dt_terms<ND>(s, e, v[0], v[1], v[2]);
-    __shared__ double sm[3][BLOCK];
+    if (valid) dt_terms<ND>(s, e, v[0], v[1], v[2]);
     block_min3(sm, v);
     if (threadIdx.x < 3) s.red[blk * 3 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final_b(DevState s)
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_partial_b(DevState s)
 {
-    const int m = blockIdx.x;
+    __shared__ double sm[3][BLOCK];
+    dt_partial_blk<ND>(s, xcd_block(), sm);
+}
+
+template <int ND> __device__ __forceinline__ void dt_final_b_body(DevState s, int m, double (*sm)[BLOCK])
+{
     bind_model<ND>(s, m);
     const int b0 = s.mblk_off[m], b1 = s.mblk_off[m + 1];
     double v[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     for (int b = b0 + threadIdx.x; b < b1; b += BLOCK)
         for (int j = 0; j < 3; ++j) v[j] = dmin(v[j], s.red[b * 3 + j]);
-    __shared__ double sm[3][BLOCK];
     block_min3(sm, v);
     if (threadIdx.x == 0) {
         double bulkm0 = 0, rho0 = 1;
         if (!s.P->is_quasi_static && s.P->fixed_dt == 0) elem0_props<ND>(s, s.e0dev[m], bulkm0, rho0);
         store_dt(s.P, s.sc, sm[0][0], sm[1][0], sm[2][0], bulkm0, rho0);
     }
+}
+
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_dt_final_b(DevState s)
+{
+    __shared__ double sm[3][BLOCK];
+    dt_final_b_body<ND>(s, blockIdx.x, sm);
 }
 
 // the bottom nodes' coord0 after a remesh (remeshing.cxx:2020-2026)
@@ -1242,9 +1256,11 @@ __device__ __forceinline__ void step_end_scalars(DevScalars *sc, int rheol_type)
 constexpr int REC = 8;
 constexpr int REC_DJ = 4, REC_M = 5, REC_TM = 6, REC_V = 7;
 
-template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
+// xbuf: this block's LDS for the records' transposed store (BLOCK * REC / 2 chunks)
+template <int ND> __device__ __forceinline__ void pass_a_body(DevState s, const FusedFlags &fl, int blk,
+                                                            bool staged, const double2 *lds, double2 *xbuf)
 {
-    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
+    BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const des_params *P = s.P;
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
@@ -1270,12 +1286,20 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_a(
     double2 r2[REC / 2];
 #pragma unroll
     for (int k = 0; k < REC / 2; ++k) r2[k] = make_double2(rec[2 * k], rec[2 * k + 1]);
-    __shared__ double2 xbuf[BLOCK * (REC / 2)];
     double2 *erec2 = reinterpret_cast<double2 *>(s.erec);
     if (!wave_store_records<REC / 2, REC / 2>(erec2, e - (int)__lane_id(), r2, xbuf + (threadIdx.x & ~63) * (REC / 2))) {
 #pragma unroll
         for (int k = 0; k < REC / 2; ++k) erec2[(long)e * (REC / 2) + k] = r2[k];
     }
+}
+
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
+{
+    __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    __shared__ double2 xbuf[BLOCK * (REC / 2)];
+    const int blk = xcd_block(s.rev);
+    const bool staged = STAGE && stage_block(s, blk, lds);
+    pass_a_body<ND>(s, fl, blk, staged, lds, xbuf);
 }
 
 // value of lane k (0..3) of this lane's quad
@@ -1290,14 +1314,13 @@ template <int K> __device__ __forceinline__ double quad_bcast(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
+template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevState s, const FusedFlags &fl, int blk)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && s.ret_count) *s.ret_count = 0;   // pass C appends
     // four lanes (a quad) per node: the quad loads an element's 64-byte
     // record with one 16-byte load per lane and shares it through DPP; every
     // lane of the quad sums the same values in the same (ascending element)
     // order, and lane 0 finishes the node
-    const int blk_ = xcd_block(s.rev);
+    const int blk_ = blk;
     const int t = blk_ * BLOCK + threadIdx.x, n = t >> 2, q = t & 3;
     if (s.nblk_model) {
         const int nb = blk_ >> 2;
@@ -1382,6 +1405,12 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
         fl.rec_mass[n] = m;
         fl.rec_volume_n[n] = vn;
     }
+}
+
+template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(DevState s, FusedFlags fl)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && s.ret_count) *s.ret_count = 0;   // pass C appends
+    pass_b_body<ND, DEPTH>(s, fl, xcd_block(s.rev));
 }
 
 template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
@@ -1472,19 +1501,27 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     }
 }
 
+template <int ND> __device__ __forceinline__ void pass_c_blk(DevState s, const FusedFlags &fl, int blk, bool staged,
+                                                           const double2 *lds)
+{
+    BIND_ELEM_AT(s, e, blk);
+    pass_c_body<ND, true>(s, fl, e, staged, lds);
+}
+
 template <int ND, int WPE, bool STAGE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
-    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
-    pass_c_body<ND, true>(s, fl, e, staged, lds);
+    __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    const int blk = xcd_block(s.rev);
+    const bool staged = STAGE && stage_block(s, blk, lds);
+    pass_c_blk<ND>(s, fl, blk, staged, lds);
 }
 
 // Pass C's Mohr-Coulomb returns (rheology.cxx:151-299, 594-635) of the 3D
 // elements that pass C deferred, from their trial stresses; the Maxwell
 // stress that pass C stored stays when its J2 is the smaller one.
-__global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
+__device__ __forceinline__ void pass_c_ret_item(const DevState &s, int k)
 {
-    const int cnt = *s.ret_count;
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < cnt; k += gridDim.x * BLOCK) {
+    {
         const int e = s.ret_elem[k];
         DevState st = s;
         if (s.eblk_model) bind_model<3>(st, s.eblk_model[e / BLOCK]);
@@ -1517,10 +1554,16 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
     }
 }
 
-// pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
-template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_d(DevState s, FusedFlags fl)
+__global__ __launch_bounds__(BLOCK) void k_pass_c_ret(DevState s)
 {
-    BIND_NODE_NB(s, n, NB);
+    const int cnt = *s.ret_count;
+    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < cnt; k += gridDim.x * BLOCK) pass_c_ret_item(s, k);
+}
+
+// pass D: node sum of dp * volume (NMD_stress, geometry.cxx:203-216)
+template <int ND, int NB, int DEPTH> __device__ __forceinline__ void pass_d_body(DevState s, const FusedFlags &fl, int blk)
+{
+    BIND_NODE_NB_AT(s, n, NB, blk);
     constexpr int NPE = Dim<ND>::NPE;
     double sum = 0;
     const int p1 = s.csr_off[n + 1];
@@ -1542,9 +1585,15 @@ template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k
     if (fl.rec_dpn) fl.rec_dpn[n] = dpn;
 }
 
-template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
+template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_d(DevState s, FusedFlags fl)
 {
-    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
+    pass_d_body<ND, NB, DEPTH>(s, fl, xcd_block(s.rev));
+}
+
+template <int ND> __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, int blk, bool staged,
+                                                            const double2 *lds)
+{
+    BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
@@ -1564,6 +1613,14 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     store_force_contrib<ND>(s, e, out);
+}
+
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
+{
+    __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    const int blk = xcd_block(s.rev);
+    const bool staged = STAGE && stage_block(s, blk, lds);
+    pass_e_body<ND>(s, fl, blk, staged, lds);
 }
 
 // wbuf: this wave's LDS for the node records' transposed store (null: the
@@ -1619,19 +1676,24 @@ template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_bo
     }
 }
 
+template <int ND, bool F32, int NB, int DEPTH> __device__ __forceinline__ void pass_f_blk(DevState s, const FusedFlags &fl,
+                                                                                     int blk, double2 *wbuf)
+{
+    BIND_NODE_NB_AT(s, n, NB, blk);
+    pass_f_body<ND, F32, DEPTH>(s, fl, n, tb_, wbuf);
+}
+
 template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f(DevState s, FusedFlags fl)
 {
     __shared__ double2 wbuf[NB * ND];
-    BIND_NODE_NB(s, n, NB);
-    pass_f_body<ND, false, DEPTH>(s, fl, n, tb_, wbuf + (threadIdx.x & ~63) * ND);
+    pass_f_blk<ND, false, NB, DEPTH>(s, fl, xcd_block(s.rev), wbuf + (threadIdx.x & ~63) * ND);
 }
 
 // pass F of the mixed-precision schedule: FP32 force contributions
 template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f32(DevState s, FusedFlags fl)
 {
     __shared__ double2 wbuf[NB * ND];
-    BIND_NODE_NB(s, n, NB);
-    pass_f_body<ND, true, DEPTH>(s, fl, n, tb_, wbuf + (threadIdx.x & ~63) * ND);
+    pass_f_blk<ND, true, NB, DEPTH>(s, fl, xcd_block(s.rev), wbuf + (threadIdx.x & ~63) * ND);
 }
 
 // ---- mixed precision (DES_MIXED): passes C and E on (pres, sdev) -------
@@ -1672,9 +1734,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState
     s.delta_plstrain[e] = (double)s.dpl32[e];
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevState s, FusedFlags fl)
+template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, const FusedFlags &fl, int blk)
 {
-    BIND_ELEM(s, e);
+    BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
@@ -1758,13 +1820,17 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevSta
     }
 }
 
+template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevState s, FusedFlags fl)
+{
+    pass_c_mixed_body<ND>(s, fl, xcd_block(s.rev));
+}
+
 // The Mohr-Coulomb returns that the mixed pass C deferred (3D), completing
 // each element as stress_update_mixed's inline return would.
-__global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
+__device__ __forceinline__ void pass_c_ret_mixed_item(const DevState &s, int k)
 {
-    const int cnt = *s.ret_count;
     const long E = s.E;
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < cnt; k += gridDim.x * BLOCK) {
+    {
         const int e = s.ret_elem[k];
         DevState st = s;
         if (s.eblk_model) bind_model<3>(st, s.eblk_model[e / BLOCK]);
@@ -1810,9 +1876,16 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
     }
 }
 
-template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
+__global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
 {
-    BIND_ELEM_STAGE_IF(s, e, lds, staged, STAGE);
+    const int cnt = *s.ret_count;
+    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < cnt; k += gridDim.x * BLOCK) pass_c_ret_mixed_item(s, k);
+}
+
+template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, const FusedFlags &fl, int blk,
+                                                                  bool staged, const double2 *lds)
+{
+    BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
@@ -1830,6 +1903,14 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     store_force_contrib_f32<ND>(s, e, out);
+}
+
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
+{
+    __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    const int blk = xcd_block(s.rev);
+    const bool staged = STAGE && stage_block(s, blk, lds);
+    pass_e_mixed_body<ND>(s, fl, blk, staged, lds);
 }
 
 // node records from the SoA state, at the start of a fused run
@@ -1870,6 +1951,171 @@ __global__ void k_step_begin(DevState s)
     s.sc->steps += 1;
 }
 
+// ---- the persistent launch: a whole run of fused steps in one kernel
+//
+// One cooperative launch holds as many workgroups as can be resident at once
+// (a multiple of 8) and runs every step of the run: the passes in the fused
+// order, each a walk over its logical blocks, separated by grid barriers
+// (every pass reads what the one before it wrote, on all XCDs).  Workgroup w
+// sits on XCD w mod 8 and walks that XCD's contiguous share of each pass's
+// logical blocks, in the directions of the per-pass launches, so each XCD
+// works on the same part of the mesh in every pass.  Up to two groups run in
+// the one launch (the 3D and the 2D models of a batch): each pass walks the
+// 3D group's blocks, then the 2D group's.  The per-block code is the
+// per-pass kernels' own, with the variants fixed: direct node-record
+// gathers, 256-thread node blocks, 8 entries in flight.
+
+struct PersistGroup {
+    int neb, nqb, nnb;          // element blocks, pass B blocks (a quad per node), node blocks
+    int plastic;                // 3D and some model is plastic: the return list runs
+    int nactive;                // traction-loaded facets
+    const int *active_facets;
+    int nmodels;                // batched group: per-model compute_dt
+};
+
+struct PersistArgs {
+    int nsteps;                 // fused steps in this launch
+    int steps0;                 // var.steps before its first step
+    int use_stored_mass;        // its first step is a run's first (mass from compute_mass)
+    int store_aux;              // its last step ends the run
+    int rev;                    // node passes walk downwards
+    PersistGroup g3, g2;
+};
+
+// logical blocks of this workgroup's walk over n3 + n2 blocks (3D group's
+// first): body3(b) for b < n3, body2(b - n3) after
+template <class F3, class F2>
+__device__ __forceinline__ void pwalk(int n3, int n2, int rev, F3 &&body3, F2 &&body2)
+{
+    const int nvb = n3 + n2, G8 = gridDim.x >> 3, x = blockIdx.x & 7;
+    const int q = nvb >> 3, r = nvb & 7, len = q + (x < r ? 1 : 0), base = x * q + min(x, r);
+#pragma nounroll
+    for (int j = blockIdx.x >> 3; j < len; j += G8) {
+        const int b = base + (rev ? len - 1 - j : j);
+        if (b < n3) body3(b);
+        else body2(b - n3);
+    }
+}
+
+// a grid-stride walk over n items
+template <class F> __device__ __forceinline__ void pitems(int n, F &&body)
+{
+#pragma nounroll
+    for (int t = blockIdx.x * BLOCK + threadIdx.x; t < n; t += gridDim.x * BLOCK) body(t);
+}
+
+// the return-list items as calls: the eigen solve's registers are not
+// reserved across the other passes' code
+__device__ __attribute__((noinline)) void pass_c_ret_call(const DevState &s, int t) { pass_c_ret_item(s, t); }
+__device__ __attribute__((noinline)) void pass_c_ret_mixed_call(const DevState &s, int t) { pass_c_ret_mixed_item(s, t); }
+
+// group state at step k of the launch: the state in device memory (read
+// after each barrier, so no pass keeps it in registers across the others),
+// with volume and volume_old swapped on odd steps (update_mesh's swap)
+__device__ __forceinline__ DevState pstate(const DevState *ps, int k)
+{
+    DevState s = *ps;
+    if (k & 1) {
+        double *t = s.volume;
+        s.volume = s.volume_old;
+        s.volume_old = t;
+    }
+    return s;
+}
+
+// a pass of both groups: CALL3 / CALL2 on the 3D / 2D group's block b, with
+// the group's state in s
+#define PWALK(n3, n2, rev, CALL3, CALL2)                                                    \
+    pwalk((n3), (n2), (rev),                                                                 \
+          [&](int b) { if constexpr (HAS3) { DevState s = pstate(ps3, k); CALL3; } },        \
+          [&](int b) { if constexpr (HAS2) { DevState s = pstate(ps2, k); CALL2; } })
+
+template <bool MIXED, bool HAS3, bool HAS2>
+__global__ __launch_bounds__(BLOCK, 2) void k_persistent(const DevState *__restrict__ ps3, const DevState *__restrict__ ps2, PersistArgs pa)
+{
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    __shared__ double2 xbuf[BLOCK * (REC / 2)];       // pass A's records, pass F's node records
+    __shared__ double sm[3][BLOCK];                   // compute_dt's block minima
+    double2 *wbuf = xbuf + (threadIdx.x & ~63) * (REC / 2);
+    const PersistGroup &g3 = pa.g3, &g2 = pa.g2;
+    const int ne3 = HAS3 ? g3.neb : 0, ne2 = HAS2 ? g2.neb : 0;
+    const int nq3 = HAS3 ? g3.nqb : 0, nq2 = HAS2 ? g2.nqb : 0;
+    const int nn3 = HAS3 ? g3.nnb : 0, nn2 = HAS2 ? g2.nnb : 0;
+    const int nf3 = HAS3 ? g3.nactive : 0, nf2 = HAS2 ? g2.nactive : 0;
+#pragma nounroll
+    for (int k = 0; k < pa.nsteps; ++k) {
+        FusedFlags fl;
+        fl.use_stored_mass = pa.use_stored_mass && k == 0;
+        fl.store_aux = pa.store_aux && k == pa.nsteps - 1;
+        PWALK(ne3, ne2, 0, pass_a_body<3>(s, fl, b, false, nullptr, xbuf),
+              pass_a_body<2>(s, fl, b, false, nullptr, xbuf));
+        if (blockIdx.x == 0 && threadIdx.x == 0) {      // pass C appends to the return lists
+            if constexpr (HAS3) if (ps3->ret_count) *ps3->ret_count = 0;
+            if constexpr (HAS2) if (ps2->ret_count) *ps2->ret_count = 0;
+        }
+        grid.sync();
+        PWALK(nq3, nq2, pa.rev, (pass_b_body<3, 8>(s, fl, b)), (pass_b_body<2, 8>(s, fl, b)));
+        grid.sync();
+        if constexpr (MIXED)
+            PWALK(ne3, ne2, 0, pass_c_mixed_body<3>(s, fl, b), pass_c_mixed_body<2>(s, fl, b));
+        else
+            PWALK(ne3, ne2, 0, pass_c_blk<3>(s, fl, b, false, nullptr), pass_c_blk<2>(s, fl, b, false, nullptr));
+        grid.sync();
+        if constexpr (HAS3) {
+            if (g3.plastic) {       // the deferred Mohr-Coulomb returns (3D only)
+                const DevState s = pstate(ps3, k);
+                const int cnt = __hip_atomic_load(s.ret_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if constexpr (MIXED) pitems(cnt, [&](int t) { pass_c_ret_mixed_call(s, t); });
+                else pitems(cnt, [&](int t) { pass_c_ret_call(s, t); });
+                grid.sync();
+            }
+        }
+        PWALK(nn3, nn2, pa.rev, (pass_d_body<3, BLOCK, 8>(s, fl, b)), (pass_d_body<2, BLOCK, 8>(s, fl, b)));
+        grid.sync();
+        if constexpr (MIXED)
+            PWALK(ne3, ne2, 0, pass_e_mixed_body<3>(s, fl, b, false, nullptr),
+                  pass_e_mixed_body<2>(s, fl, b, false, nullptr));
+        else
+            PWALK(ne3, ne2, 0, pass_e_body<3>(s, fl, b, false, nullptr), pass_e_body<2>(s, fl, b, false, nullptr));
+        grid.sync();
+        if (nf3 + nf2) {
+            pitems(nf3 + nf2, [&](int t) {
+                if constexpr (HAS3) if (t < nf3) facet_traction_item<3>(pstate(ps3, k), g3.active_facets, t);
+                if constexpr (HAS2) if (t >= nf3) facet_traction_item<2>(pstate(ps2, k), g2.active_facets, t - nf3);
+            });
+            grid.sync();
+        }
+        FusedFlags ff = fl;
+        ff.step_end = 1;
+        PWALK(nn3, nn2, pa.rev, (pass_f_blk<3, MIXED, BLOCK, 8>(s, ff, b, wbuf)),
+              (pass_f_blk<2, MIXED, BLOCK, 8>(s, ff, b, wbuf)));
+        grid.sync();
+        if ((pa.steps0 + k + 1) % 10 == 0) {          // compute_dt every tenth step
+            PWALK(ne3, ne2, 0, dt_partial_blk<3>(s, b, sm), dt_partial_blk<2>(s, b, sm));
+            grid.sync();
+            const int nm3 = HAS3 ? max(g3.nmodels, 1) : 0, nm2 = HAS2 ? max(g2.nmodels, 1) : 0;
+            for (int m = blockIdx.x; m < nm3 + nm2; m += gridDim.x) {
+                if constexpr (HAS3) {
+                    if (m < nm3) {
+                        const DevState s = pstate(ps3, k);
+                        if (g3.nmodels) dt_final_b_body<3>(s, m, sm);
+                        else dt_final_body<3>(s, ne3, sm);
+                    }
+                }
+                if constexpr (HAS2) {
+                    if (m >= nm3) {
+                        const DevState s = pstate(ps2, k);
+                        if (g2.nmodels) dt_final_b_body<2>(s, m - nm3, sm);
+                        else dt_final_body<2>(s, ne2, sm);
+                    }
+                }
+            }
+            grid.sync();
+        }
+    }
+}
+#undef PWALK
+
 }  // namespace
 
 // ================================================================ host side
@@ -1899,6 +2145,9 @@ struct des_ctx {
     int nactive = 0;
     int steps = 0;                 // host mirror of var.steps
     int avg_interval = 0;          // Output::average_interval (0: averaged fields off)
+    int launch_mode = 0;           // des_run: 0 one launch per fused pass, 1 one persistent launch
+    DevState *pstate_dev = nullptr;  // the persistent launch's copy of s in device memory
+    DevState pstate_host{};          // its host source (kept until the copy has run)
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
     std::vector<int> eh, nh;       // device element / node -> host index (model-local in a group)
     double *xfer = nullptr;        // device staging of host-order transfers, grown on demand
@@ -3775,14 +4024,101 @@ int fused_end(des_ctx *ctx, bool mixed)
     return DES_OK;
 }
 
+// ---- the persistent launch (k_persistent)
+
+template <bool M, bool H3, bool H2> const void *persistent_kernel()
+{
+    return reinterpret_cast<const void *>(&k_persistent<M, H3, H2>);
+}
+
+const void *persistent_kernel(bool mixed, bool h3, bool h2)
+{
+    if (mixed)
+        return h3 ? (h2 ? persistent_kernel<true, true, true>() : persistent_kernel<true, true, false>())
+                  : persistent_kernel<true, false, true>();
+    return h3 ? (h2 ? persistent_kernel<false, true, true>() : persistent_kernel<false, true, false>())
+              : persistent_kernel<false, false, true>();
+}
+
+PersistGroup persist_group(const des_ctx *c)
+{
+    PersistGroup g{};
+    g.neb = nblocks(c->E);
+    g.nqb = nblocks(4L * c->N);
+    g.nnb = nblocks(c->N);
+    g.plastic = c->nd == 3 && any_plastic(c);
+    g.nactive = c->nactive;
+    g.active_facets = c->active_facets;
+    g.nmodels = c->nmodels;
+    return g;
+}
+
+// A whole run of nsteps fused steps of c3 (3D) and c2 (2D) -- either may be
+// null -- in one cooperative launch on ctx's stream (ctx is one of them).
+// Both are past fused_begin and at the same step count.  The grid is as many
+// workgroups as can be resident together (hipLaunchCooperativeKernel refuses
+// more), a multiple of 8, and no more than the largest pass has blocks.
+int run_persistent(des_ctx *ctx, des_ctx *c3, des_ctx *c2, int nsteps, bool mixed)
+{
+    const void *kern = persistent_kernel(mixed, c3 != nullptr, c2 != nullptr);
+    int per_cu = 0, cus = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, 0));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    PersistArgs pa{};
+    if (c3) pa.g3 = persist_group(c3);
+    if (c2) pa.g2 = persist_group(c2);
+    const long most = std::max({(long)pa.g3.neb + pa.g2.neb, (long)pa.g3.nqb + pa.g2.nqb, (long)pa.g3.nnb + pa.g2.nnb});
+    const long G = std::min((long)per_cu * cus, (most + 7) / 8 * 8) / 8 * 8;
+    if (G < 8) return fail(ctx, DES_ERR_HIP, "persistent launch: fewer than 8 workgroups fit the device");
+    pa.nsteps = nsteps;
+    pa.steps0 = ctx->steps;
+    pa.use_stored_mass = 1;
+    pa.store_aux = 1;
+    pa.rev = g_sweep ? 1 : 0;
+    // the groups' states go to device memory, where each pass of the kernel
+    // reads what it needs (scalar loads) instead of holding both in registers
+    const DevState *ps[2] = {nullptr, nullptr};
+    int k = 0;
+    for (des_ctx *c : {c3, c2}) {
+        if (c) {
+            if (!c->pstate_dev) {      // no clearing: the copy below fills it, on this stream
+                void *p = nullptr;
+                HIPCHK(hipMalloc(&p, sizeof(DevState)));
+                c->allocs.push_back(p);
+                c->pstate_dev = static_cast<DevState *>(p);
+            }
+            c->pstate_host = c->s;
+            c->pstate_host.rev = 0;
+            HIPCHK(hipMemcpyAsync(c->pstate_dev, &c->pstate_host, sizeof(DevState), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            ps[k] = c->pstate_dev;
+        }
+        ++k;
+    }
+    void *args[] = {&ps[0], &ps[1], &pa};
+    HIPCHK(hipLaunchCooperativeKernel(kern, dim3((unsigned)G), dim3(BLOCK), args, 0, ctx->stream));
+    for (des_ctx *c : {c3, c2}) {
+        if (!c) continue;
+        if ((nsteps - 1) & 1) swap_volume(c);      // the kernel's volume swaps, steps 1 .. nsteps-1
+        c->steps += nsteps;
+        c->closing_pending = true;
+    }
+    return DES_OK;
+}
+
 int run_fused(des_ctx *ctx, int nsteps, int precision, PassTimer &tm)
 {
     if (nsteps <= 0) return precision == DES_FP64 || precision == DES_MIXED ? DES_OK : DES_ERR_INPUT;
     int rc = fused_begin(ctx, precision);
     if (rc) return rc;
     const bool mixed = precision == DES_MIXED;
-    for (int k = 0; k < nsteps; ++k)
-        if ((rc = fused_step(ctx, k, nsteps, mixed, tm))) return rc;
+    if (ctx->launch_mode == 1 && !tm.on && ctx->avg_interval == 0) {
+        if ((rc = run_persistent(ctx, ctx->nd == 3 ? ctx : nullptr, ctx->nd == 2 ? ctx : nullptr, nsteps, mixed)))
+            return rc;
+    } else {
+        for (int k = 0; k < nsteps; ++k)
+            if ((rc = fused_step(ctx, k, nsteps, mixed, tm))) return rc;
+    }
     return fused_end(ctx, mixed);
 }
 
@@ -3795,6 +4131,13 @@ int des_run(des_ctx *ctx, int nsteps, int precision)
     if (!ctx) return DES_ERR_INPUT;
     PassTimer tm(ctx, false);
     return run_fused(ctx, nsteps, precision, tm);
+}
+
+int des_set_launch_mode(des_ctx *ctx, int mode)
+{
+    if (!ctx || mode < 0 || mode > 1) return DES_ERR_INPUT;
+    ctx->launch_mode = mode;
+    return DES_OK;
 }
 
 int des_set_tuning(des_ctx *ctx, int variant)
@@ -3865,6 +4208,7 @@ struct des_batch {
     std::vector<int> group_of, index_of;   // per model
     std::vector<int> msteps;               // per model: var.steps as last set
     std::string err;
+    int launch_mode = 0;                   // des_batch_run: 0 per-pass launches, 1 one persistent launch
 };
 
 namespace {
@@ -4127,16 +4471,52 @@ int des_batch_run(des_batch *b, int nsteps, int precision)
         int rc = fused_begin(c, precision);
         if (rc) return bcheck(b, c, rc);
     }
-    for (int k = 0; k < nsteps; ++k)
-        for (size_t g = 0; g < b->groups.size(); ++g) {
-            int rc = fused_step(b->groups[g], k, nsteps, mixed, tms[g]);
-            if (rc) return bcheck(b, b->groups[g], rc);
+    bool persistent = b->launch_mode == 1;
+    des_ctx *c3 = nullptr, *c2 = nullptr;
+    for (des_ctx *c : b->groups) {
+        (c->nd == 3 ? c3 : c2) = c;
+        persistent = persistent && c->avg_interval == 0 && c->steps == b->groups[0]->steps;
+    }
+    if (persistent) {
+        // both groups in one launch on the 3D group's stream, ordered after
+        // the 2D group's earlier work and before its later work
+        des_ctx *owner = c3 ? c3 : c2;
+        hipEvent_t before = nullptr, after = nullptr;
+        if (c3 && c2) {
+            if (hipEventCreateWithFlags(&before, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&after, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(before, c2->stream) != hipSuccess ||
+                hipStreamWaitEvent(c3->stream, before, 0) != hipSuccess)
+                return bfail(b, DES_ERR_HIP, "persistent launch: stream ordering");
         }
+        int rc = run_persistent(owner, c3, c2, nsteps, mixed);
+        if (c3 && c2) {
+            if (!rc && (hipEventRecord(after, c3->stream) != hipSuccess ||
+                        hipStreamWaitEvent(c2->stream, after, 0) != hipSuccess))
+                rc = fail(owner, DES_ERR_HIP, "persistent launch: stream ordering");
+            (void)hipEventDestroy(before);
+            (void)hipEventDestroy(after);
+        }
+        if (rc) return bcheck(b, owner, rc);
+    } else {
+        for (int k = 0; k < nsteps; ++k)
+            for (size_t g = 0; g < b->groups.size(); ++g) {
+                int rc = fused_step(b->groups[g], k, nsteps, mixed, tms[g]);
+                if (rc) return bcheck(b, b->groups[g], rc);
+            }
+    }
     for (des_ctx *c : b->groups) {
         int rc = fused_end(c, mixed);
         if (rc) return bcheck(b, c, rc);
     }
     for (int &s : b->msteps) s += nsteps;
+    return DES_OK;
+}
+
+int des_batch_set_launch_mode(des_batch *b, int mode)
+{
+    if (!b || mode < 0 || mode > 1) return DES_ERR_INPUT;
+    b->launch_mode = mode;
     return DES_OK;
 }
 

@@ -31,7 +31,7 @@ EXPORTS = (
     "des_set_mesh_order", "des_set_averaging", "des_average_fields", "des_write_frame_averaged",
     "des_worst_quality_noalloc", "des_get_tuning_all", "des_set_tuning_all", "des_batch_tuning_all",
     "des_write_checkpoint", "des_remesh_finish", "des_upload_fields_device", "des_download_fields_device",
-    "des_adj_backward_device", "des_set_sweep_order",
+    "des_adj_backward_device", "des_set_sweep_order", "des_set_launch_mode", "des_batch_set_launch_mode",
 )
 
 NTUNE = 6
@@ -120,6 +120,8 @@ def load():
     lib.des_set_mesh_order.argtypes = [C.c_int]
     lib.des_set_mesh_order.restype = None
     lib.des_set_sweep_order.argtypes = [C.c_int]
+    lib.des_set_launch_mode.argtypes = [vp, C.c_int]
+    lib.des_batch_set_launch_mode.argtypes = [vp, C.c_int]
     lib.des_set_sweep_order.restype = None
     lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
     lib.des_batch_destroy.argtypes = [vp]
@@ -224,6 +226,10 @@ class DesModel:
 
     def run(self, n, precision=abi.DES_FP64):
         self._check(self.lib.des_run(self.ctx, n, precision))
+
+    def set_launch_mode(self, mode):
+        """0: one launch per fused pass (default); 1: one persistent launch per run."""
+        self._check(self.lib.des_set_launch_mode(self.ctx, int(mode)))
 
     PASS_NAMES = ("pass_a", "pass_b", "pass_c", "pass_d", "pass_e", "facet", "pass_f", "dt", "pass_c_ret")
 
@@ -485,6 +491,10 @@ class DesBatch:
 
     def run(self, n, precision=abi.DES_FP64):
         self._check(self.lib.des_batch_run(self.b, n, precision))
+
+    def set_launch_mode(self, mode):
+        """0: one launch per fused pass per group (default); 1: every model in one persistent launch."""
+        self._check(self.lib.des_batch_set_launch_mode(self.b, int(mode)))
 
     def sync(self):
         self._check(self.lib.des_batch_sync(self.b))

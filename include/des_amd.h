@@ -291,6 +291,15 @@ int des_run(des_ctx *ctx, int nsteps, int precision);
    Measurement aid with no counterpart in the reference. */
 int des_run_timed(des_ctx *ctx, int nsteps, int precision, double ms[10], int counts[10]);
 
+/* How des_run launches its steps (no counterpart in the reference): 0
+   (default) one kernel launch per fused pass, with the autotuner's variants;
+   1 one persistent cooperative launch for the whole run, its workgroups as
+   many as fit the device at once, grid barriers between the passes and the
+   variants fixed (direct node-record gathers, 256-thread node blocks).  Both
+   compute the same values in the same order: results are bitwise equal.
+   Runs with averaged fields on, and des_run_timed, launch per pass. */
+int des_set_launch_mode(des_ctx *ctx, int mode);
+
 /* Wait for all work queued on the context's stream; returns the first device
    error the step detected (dt <= 0, missing edge vector, unknown rheology).
    des_run only queues work: its device-side failures surface here (or at the
@@ -395,6 +404,10 @@ int des_batch_init_derived(des_batch *b);            /* des_init_derived for eve
 int des_batch_apply_vbcs(des_batch *b);              /* des_apply_vbcs for every model */
 int des_batch_compute_dt(des_batch *b, double *dt);  /* dt[model] */
 int des_batch_run(des_batch *b, int nsteps, int precision);
+/* des_set_launch_mode for a batch: with mode 1 every model of both dimension
+   groups runs in one persistent launch per des_batch_run (each pass walks the
+   3D models' blocks, then the 2D models'). */
+int des_batch_set_launch_mode(des_batch *b, int mode);
 int des_batch_sync(des_batch *b);
 int des_batch_worst_quality(des_batch *b, double *q, int *worst_elem);   /* per model */
 int des_batch_tuning(des_batch *b, int ndims, int *variant, double ms[DES_TUNE_MAXV]);
