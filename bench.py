@@ -292,6 +292,8 @@ def main():
     ap.add_argument("--seq-steps", type=int, default=100)
     ap.add_argument("--sweep-order", type=int, choices=(0, 1), default=1,
                     help="1: the node passes walk each XCD's blocks downwards (default); 0: upwards")
+    ap.add_argument("--force-layout", type=int, choices=(0, 1), default=0,
+                    help="FP64 force contributions: 0 node-major slots (default), 1 element-major records")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
@@ -299,6 +301,7 @@ def main():
         sys.exit(2)
     from dynearthsol_amd import lib as L
     L.set_sweep_order(args.sweep_order)
+    L.set_force_layout(args.force_layout)
     if args.workload == "ensemble":
         return bench_ensemble(args)
     if args.workload == "adjoint":
@@ -451,7 +454,7 @@ def main():
         "dtype": "f64" if primary == abi.DES_FP64 else "f32 deviator/rheology + f64 pressure/geometry/force accumulation",
         "data": "synthetic",
         "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
-                   "sweep_order": args.sweep_order,
+                   "sweep_order": args.sweep_order, "force_layout": args.force_layout,
                    "precision": "fp64" if primary == abi.DES_FP64 else "mixed",
                    "tuning": {("fp64" if p == abi.DES_FP64 else "mixed"):
                               {k: {"variant": TUNE_MEANING[k][v], "ms": [round(x, 5) for x in ms if x > 0]}

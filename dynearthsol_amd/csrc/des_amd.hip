@@ -110,9 +110,10 @@ template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState
 {
     if (staged) {
         const unsigned li = s.stage_lidx[e];
+        const lds_ptr<double2> l = (lds_ptr<double2>)lds;
 #pragma unroll
         for (int i = 0; i < Dim<ND>::NPE; ++i)
-            nrec_unpack<ND>(lds + (NREC / 2) * ((li >> (8 * i)) & 255u), x[i], v[i], T[i], aux[i]);
+            nrec_unpack<ND>(l + (NREC / 2) * ((li >> (8 * i)) & 255u), x[i], v[i], T[i], aux[i]);
     } else {
         int c[Dim<ND>::NPE];
         load_conn<ND>(s, e, c);
@@ -519,6 +520,51 @@ template <int ND> __device__ __forceinline__ void store_force_contrib_f32(const 
     }
 }
 
+// Element-major contributions (the alternative layout of the fused FP64
+// schedule): element e's record holds its contributions to its NPE nodes,
+// node i's in 16-byte chunks CH*i .. CH*i+CH-1 (3D: fx fy | fz 0, 128 bytes
+// per element; 2D: fx fy, 64 bytes with one unused chunk).  Pass E writes
+// whole records with coalesced stores (a wave's records through LDS); pass F
+// gathers node i's chunks of each element in the node's CSR order, so the
+// sums run in the same order as with node-major slots.
+template <int ND> struct FrecLayout {
+    static constexpr int CH = ND == 3 ? 2 : 1;        // chunks per node
+    static constexpr int R = ND == 3 ? 8 : 4;         // chunks per element record
+};
+
+// wbuf: this wave's LDS for the transposed store (64 x 4 chunks)
+template <int ND> __device__ __forceinline__ void store_force_rec(const DevState &s, int e,
+                                                                 const double out[Dim<ND>::NPE][ND],
+                                                                 double2 *wbuf)
+{
+    constexpr int NPE = Dim<ND>::NPE, R = FrecLayout<ND>::R;
+    double2 c[R];
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        if constexpr (ND == 3) {
+            c[2 * i] = make_double2(out[i][0], out[i][1]);
+            c[2 * i + 1] = make_double2(out[i][2], 0.0);
+        } else {
+            c[i] = make_double2(out[i][0], out[i][1]);
+        }
+    }
+    if constexpr (ND == 2) c[3] = make_double2(0.0, 0.0);
+    double2 *dst = reinterpret_cast<double2 *>(s.frec);
+    const long r0 = e - (int)__lane_id();
+    bool done;
+    if constexpr (ND == 3) {
+        const double2 lo[4] = {c[0], c[1], c[2], c[3]}, hi[4] = {c[4], c[5], c[6], c[7]};
+        done = wave_store_records<4, R>(dst, r0, lo, wbuf);
+        if (done) wave_store_records<4, R>(dst + 4, r0, hi, wbuf);
+    } else {
+        done = wave_store_records<4, R>(dst, r0, c, wbuf);
+    }
+    if (!done) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) dst[(long)e * R + k] = c[k];
+    }
+}
+
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_elem(DevState s)
 {
     BIND_ELEM(s, e);
@@ -626,14 +672,47 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_facet_traction(DevS
 
 // node part of update_force: gather, stress BCs, elastic foundation,
 // damping (fields.cxx:291-340, bc.cxx:493-566, fields.cxx:235-288)
-template <int ND, bool F32 = false, int DEPTH = 4>
+template <int ND, bool F32 = false, int DEPTH = 4, bool EREC = false>
 __device__ __forceinline__ void node_force(const DevState &s, int n, double f[ND], const double v[ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
     const des_params *P = s.P;
 #pragma unroll
     for (int d = 0; d < ND; ++d) f[d] = 0;
-    if constexpr (F32) {
+    if constexpr (EREC) {
+        // element-major records (FrecLayout), gathered in CSR (ascending host
+        // element) order
+        constexpr int CH = FrecLayout<ND>::CH, R = FrecLayout<ND>::R;
+        const double2 *fr = reinterpret_cast<const double2 *>(s.frec);
+        const int p1 = s.csr_off[n + 1];
+        int p = s.csr_off[n];
+        for (; p + DEPTH <= p1; p += DEPTH) {
+            int en[DEPTH];
+#pragma unroll
+            for (int j = 0; j < DEPTH; ++j) en[j] = s.csr_ent[p + j];
+            double c[DEPTH][ND];
+#pragma unroll
+            for (int j = 0; j < DEPTH; ++j) {
+                const long b = (long)(en[j] / NPE) * R + CH * (en[j] % NPE);
+                const double2 a = fr[b];
+                c[j][0] = a.x;
+                c[j][1] = a.y;
+                if constexpr (ND == 3) c[j][2] = reinterpret_cast<const double *>(fr + b + 1)[0];
+            }
+#pragma unroll
+            for (int j = 0; j < DEPTH; ++j)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) f[d] -= c[j][d];
+        }
+        for (; p < p1; ++p) {
+            const int ent = s.csr_ent[p];
+            const long b = (long)(ent / NPE) * R + CH * (ent % NPE);
+            const double2 a = fr[b];
+            f[0] -= a.x;
+            f[1] -= a.y;
+            if constexpr (ND == 3) f[2] -= reinterpret_cast<const double *>(fr + b + 1)[0];
+        }
+    } else if constexpr (F32) {
         // FP32 contributions (mixed precision), accumulated in FP64 in slot order
         const float *fw = reinterpret_cast<const float *>(s.fwork);
         const int p1 = s.csr_off[n + 1];
@@ -1590,8 +1669,10 @@ template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k
     pass_d_body<ND, NB, DEPTH>(s, fl, xcd_block(s.rev));
 }
 
-template <int ND> __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, int blk, bool staged,
-                                                            const double2 *lds)
+// wbuf: with EREC, this wave's LDS for the records' transposed store
+template <int ND, bool EREC = false>
+__device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds,
+                                            double2 *wbuf = nullptr)
 {
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
@@ -1612,7 +1693,8 @@ template <int ND> __device__ __forceinline__ void pass_e_body(DevState s, const 
     const double vol = s.volume[e];
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
-    store_force_contrib<ND>(s, e, out);
+    if constexpr (EREC) store_force_rec<ND>(s, e, out, wbuf);
+    else store_force_contrib<ND>(s, e, out);
 }
 
 template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
@@ -1623,10 +1705,20 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(
     pass_e_body<ND>(s, fl, blk, staged, lds);
 }
 
+// pass E writing element-major force records
+template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_er(DevState s, FusedFlags fl)
+{
+    __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    __shared__ double2 wbuf[BLOCK * 4];
+    const int blk = xcd_block(s.rev);
+    const bool staged = STAGE && stage_block(s, blk, lds);
+    pass_e_body<ND, true>(s, fl, blk, staged, lds, wbuf + (threadIdx.x & ~63) * 4);
+}
+
 // wbuf: this wave's LDS for the node records' transposed store (null: the
 // lanes store their own records)
-template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl,
-                                                                                int n, int tb_, double2 *wbuf)
+template <int ND, bool F32, int DEPTH, bool EREC = false>
+__device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl, int n, int tb_, double2 *wbuf)
 {
     const double dt = s.sc->dt;
     // the lane of each model's first node ends the step; nothing in this pass
@@ -1637,7 +1729,7 @@ template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_bo
     double f[ND], v[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = __builtin_nontemporal_load(s.vel + d * s.N + n);
-    node_force<ND, F32, DEPTH>(s, n, f, v);
+    node_force<ND, F32, DEPTH, EREC>(s, n, f, v);
     if (fl.store_aux) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) s.force[d * s.N + n] = f[d];
@@ -1676,11 +1768,11 @@ template <int ND, bool F32, int DEPTH> __device__ __forceinline__ void pass_f_bo
     }
 }
 
-template <int ND, bool F32, int NB, int DEPTH> __device__ __forceinline__ void pass_f_blk(DevState s, const FusedFlags &fl,
-                                                                                     int blk, double2 *wbuf)
+template <int ND, bool F32, int NB, int DEPTH, bool EREC = false>
+__device__ __forceinline__ void pass_f_blk(DevState s, const FusedFlags &fl, int blk, double2 *wbuf)
 {
     BIND_NODE_NB_AT(s, n, NB, blk);
-    pass_f_body<ND, F32, DEPTH>(s, fl, n, tb_, wbuf);
+    pass_f_body<ND, F32, DEPTH, EREC>(s, fl, n, tb_, wbuf);
 }
 
 template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f(DevState s, FusedFlags fl)
@@ -1690,6 +1782,13 @@ template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k
 }
 
 // pass F of the mixed-precision schedule: FP32 force contributions
+// pass F gathering element-major force records
+template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f_er(DevState s, FusedFlags fl)
+{
+    __shared__ double2 wbuf[NB * ND];
+    pass_f_blk<ND, false, NB, DEPTH, true>(s, fl, xcd_block(s.rev), wbuf + (threadIdx.x & ~63) * ND);
+}
+
 template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k_pass_f32(DevState s, FusedFlags fl)
 {
     __shared__ double2 wbuf[NB * ND];
@@ -1820,7 +1919,7 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
     }
 }
 
-template <int ND> __global__ __launch_bounds__(BLOCK) void k_pass_c_mixed(DevState s, FusedFlags fl)
+template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c_mixed(DevState s, FusedFlags fl)
 {
     pass_c_mixed_body<ND>(s, fl, xcd_block(s.rev));
 }
@@ -2431,6 +2530,7 @@ namespace {
 // results are bitwise those of the host numbering.
 int g_mesh_order = 1;    // 1: Hilbert-curve order, 0: host numbering (des_set_mesh_order)
 int g_sweep = 1;         // 1: node passes walk their blocks downwards (des_set_sweep_order)
+int g_force_layout = 0;  // fused FP64 force contributions: 0 node-major slots, 1 element-major records
 
 // Position along the Hilbert curve of the cell X[0..nd) (bits per axis):
 // J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 381
@@ -2754,6 +2854,7 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
         H2D(d_slot, slot.data(), slot.size());
         s.slot = d_slot;
         ALLOC(s.fwork, (size_t)std::max(off[N], 1) * nd);
+        ALLOC(s.frec, (size_t)std::max(E, 1) * 2 * (nd == 3 ? 8 : 4));
     }
     // boundary facets; the loaded ones get a per-node list in reference order
     // (boundary by boundary, facets ascending)
@@ -2962,6 +3063,7 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
 
 void des_set_mesh_order(int mode) { g_mesh_order = (mode >= 0 && mode <= 3) ? mode : 1; }
 void des_set_sweep_order(int mode) { g_sweep = mode ? 1 : 0; }
+void des_set_force_layout(int mode) { g_force_layout = mode ? 1 : 0; }
 
 void des_destroy(des_ctx *ctx)
 {
@@ -3722,7 +3824,7 @@ int ensure_mixed(des_ctx *ctx)
 // variant count at tuning step k) and keep the fastest mean.
 constexpr int TUNE_A = DES_TUNE_A, TUNE_C = DES_TUNE_C, TUNE_D = DES_TUNE_D, TUNE_E = DES_TUNE_E,
               TUNE_F = DES_TUNE_F, TUNE_B = DES_TUNE_B;
-constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 6, 5, 2, 5, 3}, {2, 1, 5, 2, 5, 3}};
+constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 6, 5, 2, 5, 3}, {2, 4, 5, 2, 5, 3}};
 constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 3, 0, 3, 1};
 constexpr int TUNE_STEPS = 2 * DES_TUNE_MAXV - 2, TUNE_MIN_RUN = 16;
 constexpr int PASSC_VARIANTS = 6;
@@ -3807,6 +3909,9 @@ int launch_pass_e(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
     if (mixed) {
         if (variant == 1) LAUNCH_T2(k_pass_e_mixed, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_T2(k_pass_e_mixed, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    } else if (g_force_layout) {
+        if (variant == 1) LAUNCH_T2(k_pass_e_er, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
+        else LAUNCH_T2(k_pass_e_er, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
     } else {
         if (variant == 1) LAUNCH_T2(k_pass_e, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_T2(k_pass_e, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
@@ -3818,6 +3923,8 @@ int launch_pass_f(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
 {
     if (mixed) {
         NODE_PASS_VARIANTS(k_pass_f32, ctx->s, fl);
+    } else if (g_force_layout) {
+        NODE_PASS_VARIANTS(k_pass_f_er, ctx->s, fl);
     } else {
         NODE_PASS_VARIANTS(k_pass_f, ctx->s, fl);
     }
@@ -3950,8 +4057,13 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     MARK(1);
     ARM(false);
     rc = tunable(TUNE_C, [&](int v) -> int {
-        if (mixed) {
-            LAUNCH_ND(k_pass_c_mixed, nblocks(ctx->E), ctx->s, fl);
+        if (mixed) {       // a minimum of 1..4 waves per SIMD (variants 0..3)
+            switch (v) {
+            case 1: LAUNCH_T2(k_pass_c_mixed, 2, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
+            case 2: LAUNCH_T2(k_pass_c_mixed, 3, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
+            case 3: LAUNCH_T2(k_pass_c_mixed, 4, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
+            default: LAUNCH_T2(k_pass_c_mixed, 1, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
+            }
             return DES_OK;
         }
         return launch_pass_c(ctx, fl, v);

@@ -49,6 +49,11 @@ struct MixedRet {
 };
 static_assert(sizeof(MixedRet) == 128, "one 128-byte line per record");
 
+// A pointer into LDS with its address space in the type: loads through it
+// stay LDS loads, and the compiler cannot merge them with global loads of the
+// same shape on another path into generic (flat) loads.
+template <class T> using lds_ptr = const __attribute__((address_space(3))) T *;
+
 // Every device array of one model.  Node vectors and element tensors are
 // component-major (SoA); connectivity stays AoS as in the reference.
 struct DevState {
@@ -74,6 +79,7 @@ struct DevState {
     const double *elem_visc, *elem_cohesion;      // E or null
     double *ework;                                // E x NPE scalar contributions (temperature)
     double *fwork;                                // force contributions, ND per CSR slot
+    double *frec;                                 // force contributions, element-major records (FrecLayout)
     const int *slot;                              // E x NPE: CSR position of entry e*NPE+i
     double *erec;                                 // E x 8 packed records (fused pass A -> B)
     double *escal;                                // 4 x E element scalars
@@ -148,6 +154,20 @@ template <int ND> __device__ __forceinline__ void nrec_unpack(const double2 *r, 
         v[0] = b.x; v[1] = b.y;
         T = d.x; aux = d.y;
     }
+}
+
+// the same record staged in LDS
+template <int ND> __device__ __forceinline__ void nrec_unpack(lds_ptr<double2> r, double x[ND], double v[ND],
+                                                            double &T, double &aux)
+{
+    const lds_ptr<double> d = (lds_ptr<double>)r;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        x[k] = d[k];
+        v[k] = d[ND + k];
+    }
+    T = d[2 * ND];
+    aux = d[2 * ND + 1];
 }
 
 template <int ND> __device__ __forceinline__ void load_nrec(const double *nrec, const int *c,

@@ -32,6 +32,7 @@ EXPORTS = (
     "des_worst_quality_noalloc", "des_get_tuning_all", "des_set_tuning_all", "des_batch_tuning_all",
     "des_write_checkpoint", "des_remesh_finish", "des_upload_fields_device", "des_download_fields_device",
     "des_adj_backward_device", "des_set_sweep_order", "des_set_launch_mode", "des_batch_set_launch_mode",
+    "des_set_force_layout",
 )
 
 NTUNE = 6
@@ -121,6 +122,8 @@ def load():
     lib.des_set_mesh_order.restype = None
     lib.des_set_sweep_order.argtypes = [C.c_int]
     lib.des_set_launch_mode.argtypes = [vp, C.c_int]
+    lib.des_set_force_layout.argtypes = [C.c_int]
+    lib.des_set_force_layout.restype = None
     lib.des_batch_set_launch_mode.argtypes = [vp, C.c_int]
     lib.des_set_sweep_order.restype = None
     lib.des_batch_create.argtypes = [C.c_int, C.POINTER(abi.DesParams), C.POINTER(abi.DesMesh), C.POINTER(vp)]
@@ -158,6 +161,13 @@ def set_sweep_order(mode):
     XCD walks passes B, D and F from the top of its range down, 0 upwards.
     Results do not depend on it."""
     load().des_set_sweep_order(int(mode))
+
+
+def set_force_layout(mode):
+    """Force contributions of the fused FP64 schedule, process-wide: 0
+    node-major CSR slots (default), 1 element-major records.  Results do not
+    depend on it."""
+    load().des_set_force_layout(int(mode))
 
 
 class DesModel:

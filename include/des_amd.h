@@ -168,6 +168,13 @@ void des_set_mesh_order(int mode);
    pass before it wrote last; 0 walks every pass upwards.  Each element and
    node is computed the same way in either order, so results are the same. */
 void des_set_sweep_order(int mode);
+/* Layout of the fused FP64 schedule's force contributions (process-wide; no
+   counterpart in the reference): 0 (default) node-major CSR slots, written
+   scattered by pass E and read contiguously by pass F; 1 element-major
+   records (an element's contributions to its nodes in one 128-byte line in 3D,
+   64 bytes in 2D), written coalesced by pass E and gathered by pass F.  Pass F
+   sums a node's contributions in its CSR order either way: the same results. */
+void des_set_force_layout(int mode);
 const char *des_last_error(const des_ctx *ctx);
 
 int des_upload_fields(des_ctx *ctx, const des_fields *f);
@@ -314,7 +321,8 @@ int des_sync(des_ctx *ctx);
      DES_TUNE_C:             FP64 pass C with a minimum of 1..4 waves per SIMD
                              (0..3), or of 1 or 2 with the element block's
                              node records staged in LDS (4, 5); the mixed
-                             pass C has one variant (0);
+                             pass C with a minimum of 1..4 waves per SIMD
+                             (0..3);
      DES_TUNE_D, DES_TUNE_F: node blocks of 256, 128 or 64 threads with 4
                              gathered entries in flight per node (0..2), or
                              256 threads with 8 or 16 in flight (3, 4);

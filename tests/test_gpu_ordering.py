@@ -169,3 +169,43 @@ def test_fused_run_is_independent_of_sweep_order(name, mixed):
             set_sweep_order(1)
     _same(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+@pytest.mark.parametrize("name", ["evp3", "psevp2", "bc3", "bc2", "mat3", "dynmat3", "slant3"])
+def test_fused_run_is_independent_of_force_layout(name):
+    """Node-major slots or element-major records for the force contributions
+    (des_set_force_layout): pass F sums each node's contributions in its CSR
+    order either way, so the runs are bitwise equal."""
+    from dynearthsol_amd.lib import DesModel, set_force_layout
+    m = SC.build(name)
+    print(name, m.nelem, "elements", flush=True)
+    out = []
+    for layout in (0, 1):
+        set_force_layout(layout)
+        try:
+            g = DesModel(m)
+            g.run(23)
+            out.append((g.download(), g.scalars()))
+            g.close()
+        finally:
+            set_force_layout(0)
+    _same(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+def test_batch_is_independent_of_force_layout():
+    from dynearthsol_amd.lib import DesBatch, set_force_layout
+    models = [SC.build("evp3"), SC.build("bc2"), SC.build("maxwell3", nelem=1500)]
+    out = []
+    for layout in (0, 1):
+        set_force_layout(layout)
+        try:
+            bt = DesBatch(models)
+            bt.run(21)
+            bt.sync()
+            out.append([bt.download(k) for k in range(len(models))])
+            bt.close()
+        finally:
+            set_force_layout(0)
+    for k in range(len(models)):
+        _same(out[0][k], out[1][k])

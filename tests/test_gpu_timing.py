@@ -75,7 +75,7 @@ def test_every_variant_of_every_pass_gives_identical_results(name, prec):
     ref.run(20, prec)
     a = ref.download()
     for p in TUNE_PASSES:
-        nv = 1 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
+        nv = 4 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
         for v in range(nv):
             choice = [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0, 0]
             choice[TUNE_PASSES.index(p)] = v
@@ -103,7 +103,7 @@ def test_autotuner_tunes_every_pass(prec):
     t = g.tuning_all(prec)
     print(prec, {p: (v, [round(x, 4) for x in ms]) for p, (v, ms) in t.items()}, flush=True)
     for p, (v, ms) in t.items():
-        nv = 1 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
+        nv = 4 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
         assert 0 <= v < nv, (p, v)
         assert all(x > 0 for x in ms[:nv]) and ms[v] == min(ms[:nv]), (p, ms)
     a, b = g.download(), h.download()
