@@ -56,10 +56,12 @@ ELEM_PARAM_BYTES = {"fp64": 16, "mixed": 8}     # per-element viscosity and cohe
 # (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved): (2 FETCH + WRITE)
 # * 1024.  The commands and per-kernel tables are in profiles/.
 TRAFFIC = {
-    "3d-8m": {"k_pass_a": 736113164, "k_pass_b": 1948511891, "k_pass_c": 2504874432, "k_pass_d": 644954136,
-              "k_pass_e": 2002102816, "k_pass_f": 1262318090},
-    "3d-fp64": {"k_pass_a": 92690817, "k_pass_b": 225450346, "k_pass_c": 302206372, "k_pass_d": 35101817,
-                "k_pass_e": 239278641, "k_pass_f": 154240694},
+    "3d-8m": {"k_pass_a": 818257671, "k_pass_b": 838479491, "k_pass_c": 2465313417, "k_pass_d": 372998176,
+              "k_pass_e": 2000112900, "k_pass_f": 1230323920},
+    "3d-fp64": {"k_pass_a": 92677773, "k_pass_b": 103475381, "k_pass_c": 294105556, "k_pass_d": 35105088,
+                "k_pass_e": 239736826, "k_pass_f": 152738806},
+    "3d-mixed": {"k_pass_a": 818264974, "k_pass_b": 838616656, "k_pass_c_mixed": 1681562193, "k_pass_d": 373508140,
+                 "k_pass_e_mixed": 1207732161, "k_pass_f32": 741462756},
 }
 
 WORKLOADS = {
@@ -286,6 +288,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="leave out the persistent-launch comparison (cooperative launches)")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-nelem", type=int, default=200_000)
     ap.add_argument("--ensemble-scale", type=float, default=1.0)
@@ -412,15 +416,17 @@ def main():
     g.close()
     # the same model in one persistent cooperative launch per run (grid
     # barriers between the passes), for comparison with the per-pass launches
-    gp = DesModel(m)
-    gp.set_launch_mode(1)
-    gp.run(10, primary)
-    psteps = max(1, min(args.steps, 50))
-    tp = timed(gp, psteps, primary)
-    gp.close()
-    persistent = {"steps": psteps, "ms_per_step": 1e3 * tp / psteps,
-                  "vs_per_pass_launches": (secs / args.steps) / (tp / psteps)}
-    print(f"# persistent launch: {1e3 * tp / psteps:.4f} ms/step", file=sys.stderr, flush=True)
+    persistent = None
+    if not args.no_persistent:
+        gp = DesModel(m)
+        gp.set_launch_mode(1)
+        gp.run(10, primary)
+        psteps = max(1, min(args.steps, 50))
+        tp = timed(gp, psteps, primary)
+        gp.close()
+        persistent = {"steps": psteps, "ms_per_step": 1e3 * tp / psteps,
+                      "vs_per_pass_launches": (secs / args.steps) / (tp / psteps)}
+        print(f"# persistent launch: {1e3 * tp / psteps:.4f} ms/step", file=sys.stderr, flush=True)
     tot_ms, tot_n = kernel_totals[kernel]
     proc_avg = tot_ms / tot_n
     achieved = (be * E + bn * N) / (proc_avg * 1e-3) / 1e9
