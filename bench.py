@@ -296,6 +296,9 @@ def main():
     ap.add_argument("--seq-steps", type=int, default=100)
     ap.add_argument("--sweep-order", type=int, choices=(0, 1), default=1,
                     help="1: the node passes walk each XCD's blocks downwards (default); 0: upwards")
+    ap.add_argument("--mesh", choices=("structured", "delaunay-joggled"), default="structured",
+                    help="box mesh of the 3D workloads: jittered Kuhn tetrahedra (default) or the Delaunay "
+                         "mesh of a jittered lattice (irregular node degrees, as a mesher's)")
     ap.add_argument("--force-layout", type=int, choices=(0, 1), default=0,
                     help="FP64 force contributions: 0 node-major slots (default), 1 element-major records")
     args = ap.parse_args()
@@ -318,7 +321,7 @@ def main():
     nelem = args.nelem or ne_default
     primary = abi.DES_MIXED if args.workload == "3d-mixed" else abi.DES_FP64
     t0 = time.perf_counter()
-    m = M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem)
+    m = M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem, mesh=args.mesh)
     E, N = m.nelem, m.nnode
     print(f"# {args.workload}: {E} elements, {N} nodes built in {time.perf_counter() - t0:.1f} s; "
           f"warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -443,8 +446,8 @@ def main():
         # and about 4 s per run of the restatement (39 M elem-steps/s)
         steps = args.cpu_steps or max(1, int(2.5e7 / E))
         port_steps = args.cpu_steps or max(1, int(1.6e8 / E))
-        cpu, port = cpu_baselines(M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem), steps,
-                                  port_steps)
+        cpu_model_ = M.config_benchmark(MODEL_OF[args.workload], nelem_target=nelem, mesh=args.mesh)
+        cpu, port = cpu_baselines(cpu_model_, steps, port_steps)
 
     out = {
         "metric": METRIC,
@@ -459,7 +462,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if primary == abi.DES_FP64 else "f32 deviator/rheology + f64 pressure/geometry/force accumulation",
         "data": "synthetic",
-        "config": {"workload": desc, "nelem": E, "nnode": N, "schedule": "fused six-pass",
+        "config": {"workload": desc if args.mesh == "structured" else f"{desc}, {args.mesh} mesh",
+                   "nelem": E, "nnode": N, "schedule": "fused six-pass",
                    "sweep_order": args.sweep_order, "force_layout": args.force_layout,
                    "precision": "fp64" if primary == abi.DES_FP64 else "mixed",
                    "tuning": {("fp64" if p == abi.DES_FP64 else "mixed"):
@@ -470,7 +474,8 @@ def main():
                      "bytes_per_launch": be * E + bn * N,
                      "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC.get(args.workload, {}).get(kernel) if nelem == ne_default else None},
+                     "traffic": (TRAFFIC.get(args.workload, {}).get(kernel)
+                                 if nelem == ne_default and args.mesh == "structured" else None)},
         "step_hbm": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": {k: round(v, 5) for k, v in avg.items()},

@@ -1395,10 +1395,10 @@ template <int K> __device__ __forceinline__ double quad_bcast(double v)
 
 template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevState s, const FusedFlags &fl, int blk)
 {
-    // four lanes (a quad) per node: the quad loads an element's 64-byte
-    // record with one 16-byte load per lane and shares it through DPP; every
-    // lane of the quad sums the same values in the same (ascending element)
-    // order, and lane 0 finishes the node
+    // four lanes (a quad) per node: for every entry of the node's list each
+    // lane loads the 16-byte quarter of the element's 64-byte record that
+    // holds its own sums' terms and adds them in the list's (ascending
+    // element) order; lane 0 collects the sums and finishes the node
     const int blk_ = blk;
     const int t = blk_ * BLOCK + threadIdx.x, n = t >> 2, q = t & 3;
     if (s.nblk_model) {
@@ -1410,12 +1410,12 @@ template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevStat
     }
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
-    // Lane q of the quad holds record quarter q: (c0, c1), (c2, c3), (dj, m),
-    // (tm, V).  Each sum is kept by one lane, in ascending element order:
-    // lanes 2 and 3 add their own quarter (acc = dj, m on lane 2; tm, V on
-    // lane 3); lanes 0 and 1 add the node's thermal term c_i, which sits on
-    // lane i/2, into acc.x.  Lane 0 collects the sums at the end.
+    // Record quarters: (c0, c1), (c2, c3), (dj, m), (tm, V).  Lane 0 sums the
+    // node's thermal term c_i (quarter i/2, .x or .y by i's parity) in accx;
+    // lane 1 sums dj and m, lane 2 tm and V (accx, accy); lane 3 repeats
+    // lane 2's loads and its sums are not used.
     double accx = 0, accy = 0;
+    const int qrt = q == 1 ? 2 : 3;      // the quarter of lanes 1..3
     const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
     // DEPTH entries per iteration: their records are all requested before
     // the first is summed
@@ -1433,19 +1433,18 @@ template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevStat
         double2 rq[DEPTH];
 #pragma unroll
         for (int j = 0; j < DEPTH; ++j)
-            if (en[j] >= 0) rq[j] = reinterpret_cast<const double2 *>(s.erec + (long)(en[j] / NPE) * REC)[q];
+            if (en[j] >= 0)
+                rq[j] = reinterpret_cast<const double2 *>(s.erec + (long)(en[j] / NPE) * REC)
+                    [q == 0 ? (en[j] % NPE) >> 1 : qrt];
 #pragma unroll
         for (int j = 0; j < DEPTH; ++j) {
             if (en[j] < 0) break;
-            const int i = en[j] % NPE;
-            const double sel = (i & 1) ? rq[j].y : rq[j].x;
-            const double c0 = quad_bcast<0>(sel), c1 = quad_bcast<1>(sel);
-            accx += q < 2 ? ((i >> 1) ? c1 : c0) : rq[j].x;
+            accx += (q == 0 && (en[j] % NPE) & 1) ? rq[j].y : rq[j].x;
             accy += rq[j].y;
         }
     }
-    const double tdot = accx, dj = quad_bcast<2>(accx);
-    double m = quad_bcast<2>(accy), tm = quad_bcast<3>(accx), vn = quad_bcast<3>(accy);
+    const double tdot = accx, dj = quad_bcast<1>(accx);
+    double m = quad_bcast<1>(accy), tm = quad_bcast<2>(accx), vn = quad_bcast<2>(accy);
     if (q != 0) return;
     if (!thermal) tm = 0;      // the thermal mass is summed for thermal models only
     if (fl.use_stored_mass) {
