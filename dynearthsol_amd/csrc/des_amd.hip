@@ -1372,7 +1372,7 @@ template <int ND> __device__ __forceinline__ void pass_a_body(DevState s, const 
     }
 }
 
-template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_a(DevState s, FusedFlags fl)
+template <int ND, bool STAGE, int WPE = 1> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_a(DevState s, FusedFlags fl)
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     __shared__ double2 xbuf[BLOCK * (REC / 2)];
@@ -1696,7 +1696,7 @@ __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, in
     else store_force_contrib<ND>(s, e, out);
 }
 
-template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e(DevState s, FusedFlags fl)
+template <int ND, bool STAGE, int WPE = 1> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_e(DevState s, FusedFlags fl)
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     const int blk = xcd_block(s.rev);
@@ -3823,7 +3823,7 @@ int ensure_mixed(des_ctx *ctx)
 // variant count at tuning step k) and keep the fastest mean.
 constexpr int TUNE_A = DES_TUNE_A, TUNE_C = DES_TUNE_C, TUNE_D = DES_TUNE_D, TUNE_E = DES_TUNE_E,
               TUNE_F = DES_TUNE_F, TUNE_B = DES_TUNE_B;
-constexpr int TUNE_NV[2][DES_NTUNE] = {{2, 6, 5, 2, 5, 3}, {2, 4, 5, 2, 5, 3}};
+constexpr int TUNE_NV[2][DES_NTUNE] = {{3, 6, 5, 3, 5, 3}, {3, 4, 5, 2, 5, 3}};
 constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 3, 0, 3, 1};
 constexpr int TUNE_STEPS = 2 * DES_TUNE_MAXV - 2, TUNE_MIN_RUN = 16;
 constexpr int PASSC_VARIANTS = 6;
@@ -3856,9 +3856,25 @@ int tuned_variant(const des_ctx *ctx, bool mixed, int pass)
         if (rc_) return rc_;                                                                      \
     } while (0)
 
+// kernel<ND, P2, P3> in blocks of BLOCK threads
+#define LAUNCH_E3(kern, P2, P3, grid, ...)                                                          \
+    do {                                                                                          \
+        if (ctx->nd == 3)                                                                         \
+            hipExtLaunchKernelGGL((kern<3, P2, P3>), dim3(grid), dim3(BLOCK), 0, ctx->stream,      \
+                                  ctx->ev_a, ctx->ev_b, 0, __VA_ARGS__);                          \
+        else                                                                                      \
+            hipExtLaunchKernelGGL((kern<2, P2, P3>), dim3(grid), dim3(BLOCK), 0, ctx->stream,      \
+                                  ctx->ev_a, ctx->ev_b, 0, __VA_ARGS__);                          \
+        int rc_ = check_launch(ctx);                                                              \
+        if (rc_) return rc_;                                                                      \
+    } while (0)
+
+// pass A: node records staged in LDS (0), gathered directly (1), gathered
+// directly with at least 6 waves per SIMD (2)
 int launch_pass_a(des_ctx *ctx, const FusedFlags &fl, int variant)
 {
-    if (variant == 1) LAUNCH_T2(k_pass_a, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    if (variant == 2) LAUNCH_E3(k_pass_a, false, 6, nblocks(ctx->E), ctx->s, fl);
+    else if (variant == 1) LAUNCH_T2(k_pass_a, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
     else LAUNCH_T2(k_pass_a, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
     return DES_OK;
 }
@@ -3911,8 +3927,9 @@ int launch_pass_e(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
     } else if (g_force_layout) {
         if (variant == 1) LAUNCH_T2(k_pass_e_er, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_T2(k_pass_e_er, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
-    } else {
-        if (variant == 1) LAUNCH_T2(k_pass_e, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
+    } else {     // staged (0), direct (1), staged with at least 8 waves per SIMD (2)
+        if (variant == 2) LAUNCH_E3(k_pass_e, true, 8, nblocks(ctx->E), ctx->s, fl);
+        else if (variant == 1) LAUNCH_T2(k_pass_e, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_T2(k_pass_e, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
     }
     return DES_OK;

@@ -40,7 +40,7 @@ TUNE_MAXV = 6
 TUNE_PASSES = ("pass_a", "pass_c", "pass_d", "pass_e", "pass_f", "pass_b")
 _NODE_VARIANTS = ("block256-depth4", "block128-depth4", "block64-depth4", "block256-depth8", "block256-depth16")
 TUNE_MEANING = {
-    "pass_a": ("lds-staged", "direct"), "pass_e": ("lds-staged", "direct"),
+    "pass_a": ("lds-staged", "direct", "direct waves>=6"), "pass_e": ("lds-staged", "direct", "lds-staged waves>=8"),
     "pass_c": ("waves>=1", "waves>=2", "waves>=3", "waves>=4", "lds-staged waves>=1", "lds-staged waves>=2"),
     "pass_d": _NODE_VARIANTS, "pass_f": _NODE_VARIANTS, "pass_b": ("depth4", "depth8", "depth16"),
 }

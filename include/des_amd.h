@@ -317,7 +317,9 @@ int des_sync(des_ctx *ctx);
    fused passes and their variants -- every variant performs the same
    arithmetic in the same order, so results do not depend on the choice:
      DES_TUNE_A, DES_TUNE_E: 0 the element block's node records staged in LDS,
-                             1 gathered directly;
+                             1 gathered directly, 2 pass A direct with at
+                             least 6 waves per SIMD, FP64 pass E staged with
+                             at least 8 (the mixed pass E has 0 and 1);
      DES_TUNE_C:             FP64 pass C with a minimum of 1..4 waves per SIMD
                              (0..3), or of 1 or 2 with the element block's
                              node records staged in LDS (4, 5); the mixed

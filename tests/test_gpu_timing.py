@@ -60,7 +60,8 @@ def test_autotuner_keeps_the_fastest_variant():
         g.set_tuning(6)
 
 
-VARIANTS = {"pass_a": 2, "pass_c": 6, "pass_d": 5, "pass_e": 2, "pass_f": 5, "pass_b": 3}
+VARIANTS = {"pass_a": 3, "pass_c": 6, "pass_d": 5, "pass_e": 3, "pass_f": 5, "pass_b": 3}
+MIXED_VARIANTS = dict(VARIANTS, pass_c=4, pass_e=2)
 
 
 @pytest.mark.parametrize("name,prec", [("evp3", abi.DES_FP64), ("evp3", abi.DES_MIXED), ("mat2", abi.DES_FP64),
@@ -75,7 +76,7 @@ def test_every_variant_of_every_pass_gives_identical_results(name, prec):
     ref.run(20, prec)
     a = ref.download()
     for p in TUNE_PASSES:
-        nv = 4 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
+        nv = (MIXED_VARIANTS if prec == abi.DES_MIXED else VARIANTS)[p]
         for v in range(nv):
             choice = [0, 1 if prec == abi.DES_FP64 else 0, 0, 0, 0, 0]
             choice[TUNE_PASSES.index(p)] = v
@@ -103,14 +104,14 @@ def test_autotuner_tunes_every_pass(prec):
     t = g.tuning_all(prec)
     print(prec, {p: (v, [round(x, 4) for x in ms]) for p, (v, ms) in t.items()}, flush=True)
     for p, (v, ms) in t.items():
-        nv = 4 if (p == "pass_c" and prec == abi.DES_MIXED) else VARIANTS[p]
+        nv = (MIXED_VARIANTS if prec == abi.DES_MIXED else VARIANTS)[p]
         assert 0 <= v < nv, (p, v)
         assert all(x > 0 for x in ms[:nv]) and ms[v] == min(ms[:nv]), (p, ms)
     a, b = g.download(), h.download()
     for k in a:
         assert np.array_equal(a[k], b[k]), k
     with pytest.raises(Exception):
-        g.set_tuning_all(prec, [2, 0, 0, 0, 0, 0])
+        g.set_tuning_all(prec, [3, 0, 0, 0, 0, 0])
 
 
 def test_batched_groups_tune_every_pass():
