@@ -1496,11 +1496,15 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
 {
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
-    double sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
+    double sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE], vol;
     {
+        // the volume pass A stored, recomputed from the same coordinates
+        // (the same operations on the same values: the same bits) instead of
+        // read back
         double x[NPE][ND];
         load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, aux);
-        shape_fn<ND>(x, s.volume[e], sh);
+        vol = elem_volume<ND>(x);
+        shape_fn<ND>(x, vol, sh);
     }
     double w[3];
     const bool rot = s.sc->rot_pending;
@@ -1540,7 +1544,6 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     double pl = __builtin_nontemporal_load(s.plstrain + e), dpl = __builtin_nontemporal_load(s.delta_plstrain + e);
     if (fl.rec_plstrain) fl.rec_plstrain[e] = pl;
     if (fl.rec_volume_old) fl.rec_volume_old[e] = s.volume_old[e];
-    const double vol = s.volume[e];
     bool ok = true, defer = false;
     double trial[6], old_s = 0;
     const double Tmean = node_mean<ND>(Tn);
@@ -1689,7 +1692,7 @@ __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, in
 #pragma unroll
     for (int i = 0; i < ND; ++i) s.stress[i * E + e] = st[i];
     double sh[ND][NPE], out[NPE][ND];
-    const double vol = s.volume[e];
+    const double vol = elem_volume<ND>(x);       // pass A's volume, recomputed (same bits)
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     if constexpr (EREC) store_force_rec<ND>(s, e, out, wbuf);
@@ -1839,7 +1842,7 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
     const long E = s.E;
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
     load_elem_nodes<ND>(s, e, false, nullptr, x, v, Tn, aux);
-    const double vol = s.volume[e];
+    const double vol = elem_volume<ND>(x);       // pass A's volume, recomputed (same bits)
     shape_fn<ND>(x, vol, sh);
     // velocities relative to node 0 (sum_i dN_i = 0): no cancellation of the
     // rigid-body part when the rates are rounded to FP32
@@ -1997,7 +2000,7 @@ template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, 
     double st[NSTR];
     join_stress<ND>(p, sd, st);
     double sh[ND][NPE], out[NPE][ND];
-    const double vol = s.volume[e];
+    const double vol = elem_volume<ND>(x);       // pass A's volume, recomputed (same bits)
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     store_force_contrib_f32<ND>(s, e, out);
