@@ -1857,22 +1857,24 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
         sd[k] = s.sdev[k * E + e];
         es[k] = __builtin_nontemporal_load(s.strain32 + k * E + e);
     }
+    // strain rate and spin from the same products, before the rotation, so
+    // that the velocities and shape functions are no longer held across it
+    const bool rot = s.sc->rot_pending;
+    double w[3];
+    {
+        double sr[NSTR];
+        strain_rate_spin<ND>(v, sh, sr, w, rot);
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) edot[k] = (float)sr[k];
+    }
     float rs[NSTR], re[NSTR];
 #pragma unroll
     for (int k = 0; k < NSTR; ++k) rs[k] = re[k] = 0;
-    if (s.sc->rot_pending) {
-        double w[3];
-        spin<ND>(v, sh, w);
+    if (rot) {
         const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
         const float dtp = (float)s.sc->dt_prev;
         jaumann_inc<ND, float>(sd, dtp, wf, rs);
         jaumann_inc<ND, float>(es, dtp, wf, re);
-    }
-    {
-        double sr[NSTR];
-        strain_rate<ND>(v, sh, sr);
-#pragma unroll
-        for (int k = 0; k < NSTR; ++k) edot[k] = (float)sr[k];
     }
     double edv = 0;
 #pragma unroll
