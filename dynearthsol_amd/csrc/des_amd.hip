@@ -2080,6 +2080,7 @@ struct PersistArgs {
     int nsteps;                 // fused steps in this launch
     int steps0;                 // var.steps before its first step
     int use_stored_mass;        // its first step is a run's first (mass from compute_mass)
+    int swap0;                  // its first step swaps volume and volume_old too (a continued run)
     int store_aux;              // its last step ends the run
     int rev;                    // node passes walk downwards
     PersistGroup g3, g2;
@@ -2118,7 +2119,7 @@ __device__ __attribute__((noinline)) void pass_c_ret_mixed_call(const DevState &
 __device__ __forceinline__ DevState pstate(const DevState *ps, int k)
 {
     DevState s = *ps;
-    if (k & 1) {
+    if (k & 1) {         // k counts the swaps made since the launch
         double *t = s.volume;
         s.volume = s.volume_old;
         s.volume_old = t;
@@ -2130,8 +2131,8 @@ __device__ __forceinline__ DevState pstate(const DevState *ps, int k)
 // the group's state in s
 #define PWALK(n3, n2, rev, CALL3, CALL2)                                                    \
     pwalk((n3), (n2), (rev),                                                                 \
-          [&](int b) { if constexpr (HAS3) { DevState s = pstate(ps3, k); CALL3; } },        \
-          [&](int b) { if constexpr (HAS2) { DevState s = pstate(ps2, k); CALL2; } })
+          [&](int b) { if constexpr (HAS3) { DevState s = pstate(ps3, k + pa.swap0); CALL3; } },        \
+          [&](int b) { if constexpr (HAS2) { DevState s = pstate(ps2, k + pa.swap0); CALL2; } })
 
 template <bool MIXED, bool HAS3, bool HAS2>
 __global__ __launch_bounds__(BLOCK, 2) void k_persistent(const DevState *__restrict__ ps3, const DevState *__restrict__ ps2, PersistArgs pa)
@@ -2166,7 +2167,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_persistent(const DevState *__restr
         grid.sync();
         if constexpr (HAS3) {
             if (g3.plastic) {       // the deferred Mohr-Coulomb returns (3D only)
-                const DevState s = pstate(ps3, k);
+                const DevState s = pstate(ps3, k + pa.swap0);
                 const int cnt = __hip_atomic_load(s.ret_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if constexpr (MIXED) pitems(cnt, [&](int t) { pass_c_ret_mixed_call(s, t); });
                 else pitems(cnt, [&](int t) { pass_c_ret_call(s, t); });
@@ -2183,8 +2184,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_persistent(const DevState *__restr
         grid.sync();
         if (nf3 + nf2) {
             pitems(nf3 + nf2, [&](int t) {
-                if constexpr (HAS3) if (t < nf3) facet_traction_item<3>(pstate(ps3, k), g3.active_facets, t);
-                if constexpr (HAS2) if (t >= nf3) facet_traction_item<2>(pstate(ps2, k), g2.active_facets, t - nf3);
+                if constexpr (HAS3) if (t < nf3) facet_traction_item<3>(pstate(ps3, k + pa.swap0), g3.active_facets, t);
+                if constexpr (HAS2) if (t >= nf3) facet_traction_item<2>(pstate(ps2, k + pa.swap0), g2.active_facets, t - nf3);
             });
             grid.sync();
         }
@@ -2200,14 +2201,14 @@ __global__ __launch_bounds__(BLOCK, 2) void k_persistent(const DevState *__restr
             for (int m = blockIdx.x; m < nm3 + nm2; m += gridDim.x) {
                 if constexpr (HAS3) {
                     if (m < nm3) {
-                        const DevState s = pstate(ps3, k);
+                        const DevState s = pstate(ps3, k + pa.swap0);
                         if (g3.nmodels) dt_final_b_body<3>(s, m, sm);
                         else dt_final_body<3>(s, ne3, sm);
                     }
                 }
                 if constexpr (HAS2) {
                     if (m >= nm3) {
-                        const DevState s = pstate(ps2, k);
+                        const DevState s = pstate(ps2, k + pa.swap0);
                         if (g2.nmodels) dt_final_b_body<2>(s, m - nm3, sm);
                         else dt_final_body<2>(s, ne2, sm);
                     }
@@ -2252,6 +2253,8 @@ struct des_ctx {
     DevState *pstate_dev = nullptr;  // the persistent launch's copy of s in device memory
     DevState pstate_host{};          // its host source (kept until the copy has run)
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
+    bool continuing = false;       // this run carries on from the last one's deferred tail
+    int run_precision = -1;        // the precision of the last run
     std::vector<int> eh, nh;       // device element / node -> host index (model-local in a group)
     double *xfer = nullptr;        // device staging of host-order transfers, grown on demand
     size_t xfer_cap = 0;           // its size in doubles
@@ -4019,14 +4022,24 @@ int fused_begin(des_ctx *ctx, int precision)
             if (ctx->has_visc[k] != ctx->has_visc[0] || ctx->has_coh[k] != ctx->has_coh[0])
                 return fail(ctx, DES_ERR_INPUT, "per-element viscosity / cohesion uploaded for some models only");
     }
-    int rc = flush_deferred(ctx);
-    if (rc) return rc;
+    // A run right after a run of the same precision (nothing in between
+    // touched the state, or it would have completed the deferred tail) carries
+    // on as the next step of the last one: its first step swaps the volumes,
+    // rebuilds the mass in pass B and applies the owed rotation in pass C, as
+    // every later step does, and the node records and the FP32 stress,
+    // strain and plastic-strain increment of a mixed run are current.  The
+    // results are those of one run of all the steps, without the tail's extra
+    // passes between the runs.
+    int rc;
+    if (ctx->closing_pending && ctx->run_precision != precision && (rc = flush_deferred(ctx))) return rc;
+    ctx->continuing = ctx->closing_pending;
+    ctx->run_precision = precision;
     if ((rc = ensure_retlist(ctx))) return rc;
     clear_tuning_events(ctx);          // left by a run that stopped while tuning
-    LAUNCH_ND(k_pack_nrec, nblocks(ctx->N), ctx->s);
+    if (!ctx->continuing) LAUNCH_ND(k_pack_nrec, nblocks(ctx->N), ctx->s);
     if (precision == DES_MIXED) {
         if ((rc = ensure_mixed(ctx))) return rc;
-        LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);
+        if (!ctx->continuing) LAUNCH_ND(k_to_mixed, nblocks(ctx->E), ctx->s);
     }
     return DES_OK;
 }
@@ -4037,9 +4050,9 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
 {
     int rc;
     FusedFlags fl;
-    fl.use_stored_mass = (k == 0);
+    fl.use_stored_mass = (k == 0 && !ctx->continuing);
     fl.store_aux = (k == nsteps - 1);
-    if (k > 0) swap_volume(ctx);
+    if (k > 0 || ctx->continuing) swap_volume(ctx);
     // a tuning step: the untuned passes run variant k mod their variant count
     // between HIP events
     const int pr = mixed ? 1 : 0;
@@ -4205,7 +4218,8 @@ int run_persistent(des_ctx *ctx, des_ctx *c3, des_ctx *c2, int nsteps, bool mixe
     if (G < 8) return fail(ctx, DES_ERR_HIP, "persistent launch: fewer than 8 workgroups fit the device");
     pa.nsteps = nsteps;
     pa.steps0 = ctx->steps;
-    pa.use_stored_mass = 1;
+    pa.use_stored_mass = ctx->continuing ? 0 : 1;
+    pa.swap0 = ctx->continuing ? 1 : 0;
     pa.store_aux = 1;
     pa.rev = g_sweep ? 1 : 0;
     // the groups' states go to device memory, where each pass of the kernel
@@ -4232,7 +4246,7 @@ int run_persistent(des_ctx *ctx, des_ctx *c3, des_ctx *c2, int nsteps, bool mixe
     HIPCHK(hipLaunchCooperativeKernel(kern, dim3((unsigned)G), dim3(BLOCK), args, 0, ctx->stream));
     for (des_ctx *c : {c3, c2}) {
         if (!c) continue;
-        if ((nsteps - 1) & 1) swap_volume(c);      // the kernel's volume swaps, steps 1 .. nsteps-1
+        if ((nsteps - 1 + pa.swap0) & 1) swap_volume(c);   // the swaps the kernel made
         c->steps += nsteps;
         c->closing_pending = true;
     }
@@ -4608,7 +4622,8 @@ int des_batch_run(des_batch *b, int nsteps, int precision)
     des_ctx *c3 = nullptr, *c2 = nullptr;
     for (des_ctx *c : b->groups) {
         (c->nd == 3 ? c3 : c2) = c;
-        persistent = persistent && c->avg_interval == 0 && c->steps == b->groups[0]->steps;
+        persistent = persistent && c->avg_interval == 0 && c->steps == b->groups[0]->steps &&
+                     c->continuing == b->groups[0]->continuing;
     }
     if (persistent) {
         // both groups in one launch on the 3D group's stream, ordered after
