@@ -288,7 +288,13 @@ int des_bad_mesh_quality(des_ctx *ctx, int remeshing_option, double max_boundary
 int des_step(des_ctx *ctx);
 
 /* nsteps steps of the same loop through the fused six-pass schedule, with
-   compute_dt every 10 steps as the reference (dynearthsol.cxx:349-361). */
+   compute_dt every 10 steps as the reference (dynearthsol.cxx:349-361).
+   The last step's mesh update and stress rotation stay deferred until a call
+   reads or writes the state, which completes them with the per-function
+   kernels.  A des_run right after a des_run of the same precision carries on
+   as the next step of the last one and completes them inside its first step:
+   runs of 7 + 11 + 7 steps end bitwise where one run of 25 ends (in FP64 also
+   where the same runs end with the state read between them). */
 int des_run(des_ctx *ctx, int nsteps, int precision);
 
 /* des_run with the duration of each fused pass measured by HIP events on the
