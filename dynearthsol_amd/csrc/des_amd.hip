@@ -1889,22 +1889,6 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
     bool ok = stress_update_mixed<ND, ND == 3>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p,
                                                sd, rs, es, re, edot, pl, dpl, dp, deferred, rec);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
-#pragma unroll
-    for (int k = 0; k < NSTR; ++k) __builtin_nontemporal_store(es[k], s.strain32 + k * E + e);
-    if (!deferred) {
-        s.pres[e] = p;
-#pragma unroll
-        for (int k = 0; k < NSTR; ++k) s.sdev[k * E + e] = sd[k];
-        __builtin_nontemporal_store(pl, s.plstrain + e);
-        __builtin_nontemporal_store(dpl, s.dpl32 + e);
-        s.dpressure[e] = dp;
-        s.escal[e] = dp * vol;
-    }
-    if (fl.store_aux) {
-#pragma unroll
-        for (int k = 0; k < NSTR; ++k) s.strain_rate[k * E + e] = (double)edot[k];
-        s.edvoldt[e] = edv;
-    }
     if constexpr (ND == 3) {
         // append the deferred elements, one atomic per wavefront
         const unsigned long long mask = __ballot(deferred);
@@ -1920,6 +1904,21 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
                 s.ret_mix[k] = rec;
             }
         }
+    }
+
+    if (!deferred) {
+        s.pres[e] = p;
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) s.sdev[k * E + e] = sd[k];
+        __builtin_nontemporal_store(pl, s.plstrain + e);
+        __builtin_nontemporal_store(dpl, s.dpl32 + e);
+        s.dpressure[e] = dp;
+        s.escal[e] = dp * vol;
+    }
+    if (fl.store_aux) {
+#pragma unroll
+        for (int k = 0; k < NSTR; ++k) s.strain_rate[k * E + e] = (double)edot[k];
+        s.edvoldt[e] = edv;
     }
 }
 

@@ -1371,7 +1371,9 @@ template <int ND> __device__ __forceinline__ void split_stress(const double *s, 
 
 // update_stress for one element in mixed precision (plane strain excluded).
 // rs / re: Jaumann increments of the deviator and the strain from the
-// previous step's rotation, folded into this step's single rounding.
+// previous step's rotation, folded into this step's single rounding.  The
+// updated strain es is stored to strain32 as soon as it is formed (it is not
+// held through the rheology).
 //
 // DEFER (3D): an elasto-(visco-)plastic element whose elastic trial is not
 // clearly elastic (clearly_elastic3) fills rec and sets deferred; p, sd, pl
@@ -1400,6 +1402,9 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
         de[i] = edot[i] * dtf;
         es[i] += re[i] + de[i];
     }
+    const float tr_es = trace_f<ND>(es);
+#pragma unroll
+    for (int i = 0; i < NSTR; ++i) __builtin_nontemporal_store(es[i], st.strain32 + i * (long)st.E + e);
     const float K = (float)harmonic_mean(st, P->bulk_modulus, e);
     const float G = (float)harmonic_mean(st, P->shear_modulus, e);
     float inc[NSTR];
@@ -1417,7 +1422,7 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
         for (int i = 0; i < ND; ++i) sd[i] = 2 * eta * (edot[i] - dev);
 #pragma unroll
         for (int i = ND; i < NSTR; ++i) sd[i] = 2 * eta * edot[i];
-        p = (double)K * (double)trace_f<ND>(es);
+        p = (double)K * (double)tr_es;
         break;
     }
     case DES_RH_MAXWELL: {
