@@ -1491,7 +1491,8 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
     pass_b_body<ND, DEPTH>(s, fl, xcd_block(s.rev));
 }
 
-template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
+template <int ND, bool DEFER, bool PLASTIC_ONLY = false>
+__device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
                                                                           int e, bool staged, const double2 *lds)
 {
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
@@ -1525,7 +1526,16 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
         for (int i = 0; i < ND; ++i) edot[i] += (edv - div) / ND;
         double es[NSTR];
         load_tensor_nt<ND>(s.strain, E, e, es);
-        if (rot) jaumann<ND>(es, dtp, w);
+        {
+            // rotated unconditionally and selected: a branch here keeps ~30
+            // more registers live through the rheology below
+            double er[NSTR];
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) er[i] = es[i];
+            jaumann<ND>(er, dtp, w);
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) es[i] = rot ? er[i] : es[i];
+        }
 #pragma unroll
         for (int i = 0; i < NSTR; ++i) es[i] += edot[i] * dt;
         store_tensor_nt<ND>(s.strain, E, e, es);
@@ -1549,18 +1559,15 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
     const double Tmean = node_mean<ND>(Tn);
     if (DEFER && ND == 3 && (s.P->rheol_type == DES_RH_EP || s.P->rheol_type == DES_RH_EVP)) {
         defer = plastic_deferred3(s, e, dt, Tmean, vol, s.volume_old[e], st, edot, pl, dpl, dp, trial, old_s);
-    } else {
+    } else if constexpr (!PLASTIC_ONLY) {
         // 3D plastic rheologies are deferred above: their inline return is not compiled in here
         ok = rheology_after_strain<ND, !(DEFER && ND == 3)>(s, e, dt, total_dv, Tmean, vol, s.volume_old[e], st,
                                                            syy, edot, pl, dpl, dp);
+    } else {
+        ok = false;        // launched for elasto-(visco-)plastic 3D models only
+        dp = 0;
     }
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
-    store_tensor<ND>(s.stress, E, e, st);
-    if (s.P->is_plane_strain) s.stressyy[e] = syy;
-    __builtin_nontemporal_store(pl, s.plstrain + e);
-    __builtin_nontemporal_store(dpl, s.delta_plstrain + e);
-    s.dpressure[e] = dp;
-    s.escal[e] = dp * vol;          // NMD_stress's dp * volume (geometry.cxx:206)
     if constexpr (DEFER) {
         // append the deferred elements, one atomic per wavefront
         const unsigned long long mask = __ballot(defer);
@@ -1580,21 +1587,32 @@ template <int ND, bool DEFER> __device__ __forceinline__ void pass_c_body(const 
             }
         }
     }
+    store_tensor<ND>(s.stress, E, e, st);
+    if (s.P->is_plane_strain) s.stressyy[e] = syy;
+    __builtin_nontemporal_store(pl, s.plstrain + e);
+    __builtin_nontemporal_store(dpl, s.delta_plstrain + e);
+    s.dpressure[e] = dp;
+    s.escal[e] = dp * vol;          // NMD_stress's dp * volume (geometry.cxx:206)
 }
 
-template <int ND> __device__ __forceinline__ void pass_c_blk(DevState s, const FusedFlags &fl, int blk, bool staged,
-                                                           const double2 *lds)
+template <int ND, bool PLASTIC_ONLY = false>
+__device__ __forceinline__ void pass_c_blk(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds)
 {
     BIND_ELEM_AT(s, e, blk);
-    pass_c_body<ND, true>(s, fl, e, staged, lds);
+    pass_c_body<ND, true, PLASTIC_ONLY>(s, fl, e, staged, lds);
 }
 
-template <int ND, int WPE, bool STAGE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
+// PLASTIC_ONLY: the instantiation for 3D contexts whose models are all
+// elasto-(visco-)plastic, without the other rheologies' code (whose registers
+// would otherwise set the kernel's allocation: 119 instead of 131 VGPRs, four
+// waves per SIMD instead of three)
+template <int ND, int WPE, bool STAGE, bool PLASTIC_ONLY = false>
+__global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     const int blk = xcd_block(s.rev);
     const bool staged = STAGE && stage_block(s, blk, lds);
-    pass_c_blk<ND>(s, fl, blk, staged, lds);
+    pass_c_blk<ND, PLASTIC_ONLY>(s, fl, blk, staged, lds);
 }
 
 // Pass C's Mohr-Coulomb returns (rheology.cxx:151-299, 594-635) of the 3D
@@ -1867,15 +1885,8 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
 #pragma unroll
         for (int k = 0; k < NSTR; ++k) edot[k] = (float)sr[k];
     }
-    float rs[NSTR], re[NSTR];
-#pragma unroll
-    for (int k = 0; k < NSTR; ++k) rs[k] = re[k] = 0;
-    if (rot) {
-        const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
-        const float dtp = (float)s.sc->dt_prev;
-        jaumann_inc<ND, float>(sd, dtp, wf, rs);
-        jaumann_inc<ND, float>(es, dtp, wf, re);
-    }
+    const float wf[3] = {(float)w[0], (float)w[1], (float)w[2]};
+    const float dtp = (float)s.sc->dt_prev;
     double edv = 0;
 #pragma unroll
     for (int i = 0; i < NPE; ++i) edv += aux[i];
@@ -1887,7 +1898,7 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
     bool deferred;
     MixedRet rec;
     bool ok = stress_update_mixed<ND, ND == 3>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p,
-                                               sd, rs, es, re, edot, pl, dpl, dp, deferred, rec);
+                                               sd, es, wf, dtp, rot, edot, pl, dpl, dp, deferred, rec);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     if constexpr (ND == 3) {
         // append the deferred elements, one atomic per wavefront
@@ -3954,12 +3965,34 @@ int launch_pass_f(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
     return DES_OK;
 }
 
+// whether every model of a 3D context is elasto-plastic or elasto-visco-plastic
+bool all_plastic3(const des_ctx *ctx)
+{
+    if (ctx->nd != 3) return false;
+    auto ep = [](const des_params &P) { return P.rheol_type == DES_RH_EP || P.rheol_type == DES_RH_EVP; };
+    if (!ctx->nmodels) return ep(ctx->P);
+    for (const des_params &P : ctx->hP)
+        if (!ep(P)) return false;
+    return true;
+}
+
 int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
 {
     dim3 g(nblocks(ctx->E)), b(BLOCK);
 #define PC(NDV, W, ST) \
     hipExtLaunchKernelGGL((k_pass_c<NDV, W, ST>), g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s, fl)
-    if (ctx->nd == 3) {
+#define PCP(W, ST) \
+    hipExtLaunchKernelGGL((k_pass_c<3, W, ST, true>), g, b, 0, ctx->stream, ctx->ev_a, ctx->ev_b, 0, ctx->s, fl)
+    if (ctx->nd == 3 && all_plastic3(ctx)) {
+        switch (variant) {
+        case 0: PCP(1, false); break;
+        case 1: PCP(2, false); break;
+        case 2: PCP(3, false); break;
+        case 3: PCP(4, false); break;
+        case 4: PCP(1, true); break;
+        default: PCP(2, true); break;
+        }
+    } else if (ctx->nd == 3) {
         switch (variant) {
         case 0: PC(3, 1, false); break;
         case 1: PC(3, 2, false); break;
@@ -3979,6 +4012,7 @@ int launch_pass_c(des_ctx *ctx, const FusedFlags &fl, int variant)
         }
     }
 #undef PC
+#undef PCP
     return check_launch(ctx);
 }
 

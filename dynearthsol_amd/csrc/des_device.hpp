@@ -1370,10 +1370,12 @@ template <int ND> __device__ __forceinline__ void split_stress(const double *s, 
 }
 
 // update_stress for one element in mixed precision (plane strain excluded).
-// rs / re: Jaumann increments of the deviator and the strain from the
-// previous step's rotation, folded into this step's single rounding.  The
-// updated strain es is stored to strain32 as soon as it is formed (it is not
-// held through the rheology).
+// wf, dtp, rot: the spin and dt of the previous step's Jaumann rotation,
+// owed when rot is set.  Its increments of the strain (re) and of the deviator
+// (rs) are folded into this step's single rounding, each formed where it is
+// used, and the updated strain es is stored to strain32 as soon as it is
+// formed: carried from the caller instead, they kept ~45 more registers live
+// through the rheology (4 waves per SIMD instead of 3).
 //
 // DEFER (3D): an elasto-(visco-)plastic element whose elastic trial is not
 // clearly elastic (clearly_elastic3) fills rec and sets deferred; p, sd, pl
@@ -1383,7 +1385,7 @@ template <int ND> __device__ __forceinline__ void split_stress(const double *s, 
 template <int ND, bool DEFER = false>
 __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf, float edvoldt,
                                                     float Tmean, double vol, double vol_old, double &p,
-                                                    float *sd, const float *rs, float *es, const float *re,
+                                                    float *sd, float *es, const float *wf, float dtp, bool rot,
                                                     float *edot, double &pl, float &dpl, double &dpressure,
                                                     bool &deferred, MixedRet &rec)
 {
@@ -1397,10 +1399,14 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
         for (int i = 0; i < ND; ++i) edot[i] += (edvoldt - div) / ND;
     }
     float de[NSTR];
+    {
+        float re[NSTR];
+        jaumann_inc<ND, float>(es, dtp, wf, re);
 #pragma unroll
-    for (int i = 0; i < NSTR; ++i) {
-        de[i] = edot[i] * dtf;
-        es[i] += re[i] + de[i];
+        for (int i = 0; i < NSTR; ++i) {
+            de[i] = edot[i] * dtf;
+            es[i] += (rot ? re[i] : 0.f) + de[i];
+        }
     }
     const float tr_es = trace_f<ND>(es);
 #pragma unroll
@@ -1408,6 +1414,10 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
     const float K = (float)harmonic_mean(st, P->bulk_modulus, e);
     const float G = (float)harmonic_mean(st, P->shear_modulus, e);
     float inc[NSTR];
+    float rs[NSTR];
+    jaumann_inc<ND, float>(sd, dtp, wf, rs);
+#pragma unroll
+    for (int i = 0; i < NSTR; ++i) rs[i] = rot ? rs[i] : 0.f;
     bool ok = true;
     switch (P->rheol_type) {
     case DES_RH_ELASTIC:
