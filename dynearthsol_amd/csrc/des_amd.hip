@@ -1418,17 +1418,22 @@ template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevStat
     const int qrt = q == 1 ? 2 : 3;      // the quarter of lanes 1..3
     const int p0 = s.csr_off[n], p1 = s.csr_off[n + 1];
     // DEPTH entries per iteration: their records are all requested before
-    // the first is summed
+    // the first is summed, and the next iteration's entries are requested
+    // before these records are summed
+    int mine[DEPTH / 4];
+#pragma unroll
+    for (int g = 0; g < DEPTH / 4; ++g) {
+        const int pp = p0 + 4 * g + q;
+        mine[g] = pp < p1 ? s.csr_ent[pp] : -1;
+    }
     for (int p = p0; p < p1; p += DEPTH) {
         int en[DEPTH];
 #pragma unroll
         for (int g = 0; g < DEPTH / 4; ++g) {
-            const int pp = p + 4 * g + q;
-            const int mine = pp < p1 ? s.csr_ent[pp] : -1;
-            en[4 * g + 0] = quad_bcast<0>(mine);
-            en[4 * g + 1] = quad_bcast<1>(mine);
-            en[4 * g + 2] = quad_bcast<2>(mine);
-            en[4 * g + 3] = quad_bcast<3>(mine);
+            en[4 * g + 0] = quad_bcast<0>(mine[g]);
+            en[4 * g + 1] = quad_bcast<1>(mine[g]);
+            en[4 * g + 2] = quad_bcast<2>(mine[g]);
+            en[4 * g + 3] = quad_bcast<3>(mine[g]);
         }
         double2 rq[DEPTH];
 #pragma unroll
@@ -1436,6 +1441,11 @@ template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevStat
             if (en[j] >= 0)
                 rq[j] = reinterpret_cast<const double2 *>(s.erec + (long)(en[j] / NPE) * REC)
                     [q == 0 ? (en[j] % NPE) >> 1 : qrt];
+#pragma unroll
+        for (int g = 0; g < DEPTH / 4; ++g) {
+            const int pp = p + DEPTH + 4 * g + q;
+            mine[g] = pp < p1 ? s.csr_ent[pp] : -1;
+        }
 #pragma unroll
         for (int j = 0; j < DEPTH; ++j) {
             if (en[j] < 0) break;
