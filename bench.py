@@ -299,8 +299,9 @@ def main():
     ap.add_argument("--mesh", choices=("structured", "delaunay-joggled"), default="structured",
                     help="box mesh of the 3D workloads: jittered Kuhn tetrahedra (default) or the Delaunay "
                          "mesh of a jittered lattice (irregular node degrees, as a mesher's)")
-    ap.add_argument("--force-layout", type=int, choices=(0, 1), default=0,
-                    help="FP64 force contributions: 0 node-major slots (default), 1 element-major records")
+    ap.add_argument("--force-layout", type=int, choices=(0, 1, 2), default=0,
+                    help="FP64 force contributions: 0 node-major slots found from list start + one-byte "
+                         "position (default), 1 element-major records, 2 node-major slots found by full positions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "

@@ -88,7 +88,7 @@ __device__ __forceinline__ int xcd_block(int rev = 0)
 // distinct nodes (a compact patch of the curve-ordered mesh) are loaded once,
 // four lanes per 64-byte record, instead of four scattered records per
 // element.  Every lane of the block calls this before any lane returns.
-__device__ __forceinline__ bool stage_block(const DevState &s, int blk, double2 *lds)
+__device__ __forceinline__ bool stage_block(const DevState &s, int blk, double2 *lds, int *loff = nullptr)
 {
     if (!s.stage) return false;
     const int2 st = s.stage[blk];
@@ -96,6 +96,8 @@ __device__ __forceinline__ bool stage_block(const DevState &s, int blk, double2 
     const double2 *src = reinterpret_cast<const double2 *>(s.nrec);
     for (int k = threadIdx.x; k < (NREC / 2) * st.y; k += BLOCK)
         lds[k] = src[(long)s.stage_node[st.x + k / (NREC / 2)] * (NREC / 2) + k % (NREC / 2)];
+    if (loff && s.srank)       // and where each staged node's list starts
+        for (int k = threadIdx.x; k < st.y; k += BLOCK) loff[k] = s.csr_off[s.stage_node[st.x + k]];
     __syncthreads();
     return true;
 }
@@ -493,13 +495,43 @@ template <int ND> __device__ __forceinline__ void force_contrib(const DevState &
 // run per node.  With the mesh in curve order an element's slots are close,
 // and the scattered stores cost the element pass less than the contiguous
 // gather saves (measured against 96-byte element-major records).
-template <int ND> __device__ __forceinline__ void store_force_contrib(const DevState &s, int e,
+// The CSR positions (force slots) of element e's NPE entries: the start of
+// each node's list plus the entry's position in it, one byte each (srank),
+// with the list starts from the staged block's LDS (loff) or from csr_off;
+// the full 4-byte positions (slot) when some list is longer than 256, or
+// always with RANK false (the mixed schedule: 12-byte slots, measured faster
+// with the plain positions).
+template <int ND, bool RANK = true>
+__device__ __forceinline__ void force_slots(const DevState &s, int e, bool staged, const int *loff,
+                                            int pos[Dim<ND>::NPE])
+{
+    constexpr int NPE = Dim<ND>::NPE;
+    if (RANK && s.srank) {
+        const unsigned r = s.srank[e];
+        if (staged) {
+            const unsigned li = s.stage_lidx[e];
+            const lds_ptr<int> l = (lds_ptr<int>)loff;
+#pragma unroll
+            for (int i = 0; i < NPE; ++i) pos[i] = l[(li >> (8 * i)) & 255u] + (int)((r >> (8 * i)) & 255u);
+        } else {
+            int c[NPE];
+            load_conn<ND>(s, e, c);
+#pragma unroll
+            for (int i = 0; i < NPE; ++i) pos[i] = s.csr_off[c[i]] + (int)((r >> (8 * i)) & 255u);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NPE; ++i) pos[i] = s.slot[e * NPE + i];
+    }
+}
+
+template <int ND> __device__ __forceinline__ void store_force_contrib(const DevState &s, const int pos[Dim<ND>::NPE],
                                                                     const double out[Dim<ND>::NPE][ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
 #pragma unroll
     for (int i = 0; i < NPE; ++i) {
-        double *w = s.fwork + (long)s.slot[e * NPE + i] * ND;
+        double *w = s.fwork + (long)pos[i] * ND;
 #pragma unroll
         for (int d = 0; d < ND; ++d) w[d] = out[i][d];
     }
@@ -507,14 +539,15 @@ template <int ND> __device__ __forceinline__ void store_force_contrib(const DevS
 
 // The mixed-precision schedule's contributions: the same slots holding FP32
 // values (12 bytes in 3D), summed in FP64 by the node pass.
-template <int ND> __device__ __forceinline__ void store_force_contrib_f32(const DevState &s, int e,
+template <int ND> __device__ __forceinline__ void store_force_contrib_f32(const DevState &s,
+                                                                        const int pos[Dim<ND>::NPE],
                                                                         const double out[Dim<ND>::NPE][ND])
 {
     constexpr int NPE = Dim<ND>::NPE;
     float *fw = reinterpret_cast<float *>(s.fwork);
 #pragma unroll
     for (int i = 0; i < NPE; ++i) {
-        float *w = fw + (long)s.slot[e * NPE + i] * ND;
+        float *w = fw + (long)pos[i] * ND;
 #pragma unroll
         for (int d = 0; d < ND; ++d) w[d] = (float)out[i][d];
     }
@@ -575,7 +608,9 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_force_elem(DevState
     load_shp<ND>(s, e, sh);
     load_tensor<ND>(s.stress, s.E, e, st);
     force_contrib<ND>(s, e, s.volume[e], sh, st, elem_mean<ND>(s.T, c), out);
-    store_force_contrib<ND>(s, e, out);
+    int pos[NPE];
+    force_slots<ND>(s, e, false, nullptr, pos);
+    store_force_contrib<ND>(s, pos, out);
 }
 
 // PREM pressure tables (matprops.cxx:11-99)
@@ -1700,9 +1735,10 @@ template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k
 }
 
 // wbuf: with EREC, this wave's LDS for the records' transposed store
+// loff: with staged, the staged nodes' list starts (stage_block)
 template <int ND, bool EREC = false>
 __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds,
-                                            double2 *wbuf = nullptr)
+                                            const int *loff, double2 *wbuf = nullptr)
 {
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
@@ -1723,16 +1759,22 @@ __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, in
     const double vol = elem_volume<ND>(x);       // pass A's volume, recomputed (same bits)
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
-    if constexpr (EREC) store_force_rec<ND>(s, e, out, wbuf);
-    else store_force_contrib<ND>(s, e, out);
+    if constexpr (EREC) {
+        store_force_rec<ND>(s, e, out, wbuf);
+    } else {
+        int pos[NPE];
+        force_slots<ND>(s, e, staged, loff, pos);
+        store_force_contrib<ND>(s, pos, out);
+    }
 }
 
 template <int ND, bool STAGE, int WPE = 1> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_e(DevState s, FusedFlags fl)
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    __shared__ int loff[STAGE ? STAGE_MAX : 1];
     const int blk = xcd_block(s.rev);
-    const bool staged = STAGE && stage_block(s, blk, lds);
-    pass_e_body<ND>(s, fl, blk, staged, lds);
+    const bool staged = STAGE && stage_block(s, blk, lds, loff);
+    pass_e_body<ND>(s, fl, blk, staged, lds, loff);
 }
 
 // pass E writing element-major force records
@@ -1742,7 +1784,7 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_
     __shared__ double2 wbuf[BLOCK * 4];
     const int blk = xcd_block(s.rev);
     const bool staged = STAGE && stage_block(s, blk, lds);
-    pass_e_body<ND, true>(s, fl, blk, staged, lds, wbuf + (threadIdx.x & ~63) * 4);
+    pass_e_body<ND, true>(s, fl, blk, staged, lds, nullptr, wbuf + (threadIdx.x & ~63) * 4);
 }
 
 // wbuf: this wave's LDS for the node records' transposed store (null: the
@@ -2025,7 +2067,9 @@ template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, 
     const double vol = elem_volume<ND>(x);       // pass A's volume, recomputed (same bits)
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
-    store_force_contrib_f32<ND>(s, e, out);
+    int pos[NPE];
+    force_slots<ND, false>(s, e, staged, nullptr, pos);
+    store_force_contrib_f32<ND>(s, pos, out);
 }
 
 template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
@@ -2200,7 +2244,8 @@ __global__ __launch_bounds__(BLOCK, 2) void k_persistent(const DevState *__restr
             PWALK(ne3, ne2, 0, pass_e_mixed_body<3>(s, fl, b, false, nullptr),
                   pass_e_mixed_body<2>(s, fl, b, false, nullptr));
         else
-            PWALK(ne3, ne2, 0, pass_e_body<3>(s, fl, b, false, nullptr), pass_e_body<2>(s, fl, b, false, nullptr));
+            PWALK(ne3, ne2, 0, pass_e_body<3>(s, fl, b, false, nullptr, nullptr),
+                  pass_e_body<2>(s, fl, b, false, nullptr, nullptr));
         grid.sync();
         if (nf3 + nf2) {
             pitems(nf3 + nf2, [&](int t) {
@@ -2275,6 +2320,7 @@ struct des_ctx {
     bool closing_pending = false;  // des_run left volume/mass/shape functions/rotation deferred
     bool continuing = false;       // this run carries on from the last one's deferred tail
     int run_precision = -1;        // the precision of the last run
+    const unsigned *srank = nullptr;   // the entries' positions in their node lists (DevState::srank)
     std::vector<int> eh, nh;       // device element / node -> host index (model-local in a group)
     double *xfer = nullptr;        // device staging of host-order transfers, grown on demand
     size_t xfer_cap = 0;           // its size in doubles
@@ -2557,7 +2603,8 @@ namespace {
 // results are bitwise those of the host numbering.
 int g_mesh_order = 1;    // 1: Hilbert-curve order, 0: host numbering (des_set_mesh_order)
 int g_sweep = 1;         // 1: node passes walk their blocks downwards (des_set_sweep_order)
-int g_force_layout = 0;  // fused FP64 force contributions: 0 node-major slots, 1 element-major records
+int g_force_layout = 0;  // fused FP64 force contributions: 0 node-major slots, 1 element-major records,
+                         // 2 node-major slots addressed by their full 4-byte positions
 
 // Position along the Hilbert curve of the cell X[0..nd) (bits per axis):
 // J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 381
@@ -2880,6 +2927,21 @@ int create_impl(des_ctx *ctx, const des_params *p, const des_mesh *m, const Grou
         ALLOC(d_slot, slot.size());
         H2D(d_slot, slot.data(), slot.size());
         s.slot = d_slot;
+        // and each entry's position in its node's list, one byte per entry,
+        // when no list is longer than 256 (4 bytes per element instead of 16)
+        int maxdeg = 0;
+        for (int n = 0; n < N; ++n) maxdeg = std::max(maxdeg, off[n + 1] - off[n]);
+        if (maxdeg <= 256 && E > 0) {
+            std::vector<unsigned> rk((size_t)E, 0u);
+            for (int n = 0; n < N; ++n)
+                for (int q = off[n]; q < off[n + 1]; ++q)
+                    rk[ent[q] / npe] |= (unsigned)(q - off[n]) << (8 * (ent[q] % npe));
+            unsigned *d_rk;
+            ALLOC(d_rk, rk.size());
+            H2D(d_rk, rk.data(), rk.size());
+            s.srank = d_rk;
+            ctx->srank = d_rk;
+        }
         ALLOC(s.fwork, (size_t)std::max(off[N], 1) * nd);
         ALLOC(s.frec, (size_t)std::max(E, 1) * 2 * (nd == 3 ? 8 : 4));
     }
@@ -3090,7 +3152,7 @@ int des_create(const des_params *p, const des_mesh *m, des_ctx **out)
 
 void des_set_mesh_order(int mode) { g_mesh_order = (mode >= 0 && mode <= 3) ? mode : 1; }
 void des_set_sweep_order(int mode) { g_sweep = mode ? 1 : 0; }
-void des_set_force_layout(int mode) { g_force_layout = mode ? 1 : 0; }
+void des_set_force_layout(int mode) { g_force_layout = mode == 1 || mode == 2 ? mode : 0; }
 
 void des_destroy(des_ctx *ctx)
 {
@@ -3952,7 +4014,7 @@ int launch_pass_e(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
     if (mixed) {
         if (variant == 1) LAUNCH_T2(k_pass_e_mixed, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_T2(k_pass_e_mixed, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
-    } else if (g_force_layout) {
+    } else if (g_force_layout == 1) {
         if (variant == 1) LAUNCH_T2(k_pass_e_er, false, BLOCK, nblocks(ctx->E), ctx->s, fl);
         else LAUNCH_T2(k_pass_e_er, true, BLOCK, nblocks(ctx->E), ctx->s, fl);
     } else {     // staged (0), direct (1), staged with at least 8 waves per SIMD (2)
@@ -3967,7 +4029,7 @@ int launch_pass_f(des_ctx *ctx, const FusedFlags &fl, bool mixed, int variant)
 {
     if (mixed) {
         NODE_PASS_VARIANTS(k_pass_f32, ctx->s, fl);
-    } else if (g_force_layout) {
+    } else if (g_force_layout == 1) {
         NODE_PASS_VARIANTS(k_pass_f_er, ctx->s, fl);
     } else {
         NODE_PASS_VARIANTS(k_pass_f, ctx->s, fl);
@@ -4078,6 +4140,7 @@ int fused_begin(des_ctx *ctx, int precision)
     ctx->continuing = ctx->closing_pending;
     ctx->run_precision = precision;
     if ((rc = ensure_retlist(ctx))) return rc;
+    ctx->s.srank = g_force_layout == 2 ? nullptr : ctx->srank;
     clear_tuning_events(ctx);          // left by a run that stopped while tuning
     if (!ctx->continuing) LAUNCH_ND(k_pack_nrec, nblocks(ctx->N), ctx->s);
     if (precision == DES_MIXED) {

@@ -81,6 +81,8 @@ struct DevState {
     double *fwork;                                // force contributions, ND per CSR slot
     double *frec;                                 // force contributions, element-major records (FrecLayout)
     const int *slot;                              // E x NPE: CSR position of entry e*NPE+i
+    const unsigned *srank;                        // E: byte i = position of entry e*NPE+i in its
+                                                  // node's list (null: a list longer than 256)
     double *erec;                                 // E x 8 packed records (fused pass A -> B)
     double *escal;                                // 4 x E element scalars
     double *ftrac;                                // nbf x ND facet tractions

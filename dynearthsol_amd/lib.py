@@ -165,8 +165,10 @@ def set_sweep_order(mode):
 
 def set_force_layout(mode):
     """Force contributions of the fused FP64 schedule, process-wide: 0
-    node-major CSR slots (default), 1 element-major records.  Results do not
-    depend on it."""
+    node-major CSR slots addressed by list start + one-byte position
+    (default), 1 element-major records, 2 node-major slots addressed by their
+    full positions (as the mixed schedule's FP32 slots always are).  Results
+    do not depend on it."""
     load().des_set_force_layout(int(mode))
 
 

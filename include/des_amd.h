@@ -170,10 +170,14 @@ void des_set_mesh_order(int mode);
 void des_set_sweep_order(int mode);
 /* Layout of the fused FP64 schedule's force contributions (process-wide; no
    counterpart in the reference): 0 (default) node-major CSR slots, written
-   scattered by pass E and read contiguously by pass F; 1 element-major
-   records (an element's contributions to its nodes in one 128-byte line in 3D,
-   64 bytes in 2D), written coalesced by pass E and gathered by pass F.  Pass F
-   sums a node's contributions in its CSR order either way: the same results. */
+   scattered by pass E and read contiguously by pass F, pass E finding each
+   slot from the start of the node's list and the entry's position in it (one
+   byte per entry; full 4-byte positions when a node list is longer than 256);
+   1 element-major records (an element's contributions to its nodes in one
+   128-byte line in 3D, 64 bytes in 2D), written coalesced by pass E and
+   gathered by pass F; 2 the node-major slots of 0 always found by their full
+   positions (as the mixed schedule's FP32 slots always are).  Pass F sums a
+   node's contributions in its CSR order every way: the same results. */
 void des_set_force_layout(int mode);
 const char *des_last_error(const des_ctx *ctx);
 

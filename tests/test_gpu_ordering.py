@@ -173,14 +173,16 @@ def test_fused_run_is_independent_of_sweep_order(name, mixed):
 
 @pytest.mark.parametrize("name", ["evp3", "psevp2", "bc3", "bc2", "mat3", "dynmat3", "slant3"])
 def test_fused_run_is_independent_of_force_layout(name):
-    """Node-major slots or element-major records for the force contributions
-    (des_set_force_layout): pass F sums each node's contributions in its CSR
-    order either way, so the runs are bitwise equal."""
+    """Node-major slots addressed by list start + one-byte position (0), the
+    same slots by their full 4-byte positions (2) or element-major records (1)
+    for the force contributions (des_set_force_layout): pass F sums each
+    node's contributions in its CSR order every way, so the runs are bitwise
+    equal."""
     from dynearthsol_amd.lib import DesModel, set_force_layout
     m = SC.build(name)
     print(name, m.nelem, "elements", flush=True)
     out = []
-    for layout in (0, 1):
+    for layout in (0, 1, 2):
         set_force_layout(layout)
         try:
             g = DesModel(m)
@@ -189,15 +191,16 @@ def test_fused_run_is_independent_of_force_layout(name):
             g.close()
         finally:
             set_force_layout(0)
-    _same(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
+    for k in (1, 2):
+        _same(out[0][0], out[k][0])
+        assert out[0][1] == out[k][1]
 
 
 def test_batch_is_independent_of_force_layout():
     from dynearthsol_amd.lib import DesBatch, set_force_layout
     models = [SC.build("evp3"), SC.build("bc2"), SC.build("maxwell3", nelem=1500)]
     out = []
-    for layout in (0, 1):
+    for layout in (0, 1, 2):
         set_force_layout(layout)
         try:
             bt = DesBatch(models)
@@ -209,3 +212,4 @@ def test_batch_is_independent_of_force_layout():
             set_force_layout(0)
     for k in range(len(models)):
         _same(out[0][k], out[1][k])
+        _same(out[0][k], out[2][k])
