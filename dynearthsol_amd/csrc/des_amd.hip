@@ -1738,18 +1738,24 @@ template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k
 // loff: with staged, the staged nodes' list starts (stage_block)
 template <int ND, bool EREC = false>
 __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds,
-                                            const int *loff, double2 *wbuf = nullptr)
+                                            const int *loff, double2 *wbuf = nullptr, const double *pre = nullptr)
 {
+    // pre: the element's stress and dpressure when the caller loaded them
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
     load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn);
     double st[NSTR];
-    load_tensor<ND>(s.stress, E, e, st);
+    if (pre) {
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) st[i] = pre[i];
+    } else {
+        load_tensor<ND>(s.stress, E, e, st);
+    }
     {   // NMD_stress's element side (geometry.cxx:221-235), dp_nd from the records
         const double dp_el = node_mean<ND>(dpn);
-        const double dp_orig = s.dpressure[e];
+        const double dp_orig = pre ? pre[NSTR] : s.dpressure[e];
 #pragma unroll
         for (int i = 0; i < ND; ++i) st[i] += (-dp_orig + dp_el) / ND;
     }
@@ -1773,8 +1779,21 @@ template <int ND, bool STAGE, int WPE = 1> __global__ __launch_bounds__(BLOCK, W
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     __shared__ int loff[STAGE ? STAGE_MAX : 1];
     const int blk = xcd_block(s.rev);
+    // the element's stress and pressure change do not depend on the staged
+    // records: requested before the block's staging barrier, so that their
+    // HBM latency overlaps the staging's (the compiler keeps loads written
+    // after a barrier after it)
+    constexpr int NSTR = Dim<ND>::NSTR;
+    double pre[NSTR + 1];
+    if constexpr (STAGE) {
+        const int e0 = blk * BLOCK + threadIdx.x;
+        if (s.eblk_model ? e0 < s.eblk_lim[blk] : e0 < s.E) {
+            load_tensor<ND>(s.stress, s.E, e0, pre);
+            pre[NSTR] = s.dpressure[e0];
+        }
+    }
     const bool staged = STAGE && stage_block(s, blk, lds, loff);
-    pass_e_body<ND>(s, fl, blk, staged, lds, loff);
+    pass_e_body<ND>(s, fl, blk, staged, lds, loff, nullptr, STAGE ? pre : nullptr);
 }
 
 // pass E writing element-major force records
@@ -2048,7 +2067,8 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
 }
 
 template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, const FusedFlags &fl, int blk,
-                                                                  bool staged, const double2 *lds)
+                                                                  bool staged, const double2 *lds,
+                                                                  const double *pre = nullptr, const float *psd = nullptr)
 {
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
@@ -2056,11 +2076,11 @@ template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, 
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
     load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn);
     const double dp_el = node_mean<ND>(dpn);
-    double p = s.pres[e] + (-s.dpressure[e] + dp_el) / ND;   // NMD_stress on the diagonal mean
+    double p = (pre ? pre[0] : s.pres[e]) + (-(pre ? pre[1] : s.dpressure[e]) + dp_el) / ND;
     s.pres[e] = p;
     float sd[NSTR];
 #pragma unroll
-    for (int k = 0; k < NSTR; ++k) sd[k] = s.sdev[k * E + e];
+    for (int k = 0; k < NSTR; ++k) sd[k] = psd ? psd[k] : s.sdev[k * E + e];
     double st[NSTR];
     join_stress<ND>(p, sd, st);
     double sh[ND][NPE], out[NPE][ND];
@@ -2076,8 +2096,22 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     const int blk = xcd_block(s.rev);
+    // pressure, pressure change and deviator requested before the staging
+    // barrier (as in k_pass_e)
+    constexpr int NSTR = Dim<ND>::NSTR;
+    double pre[2];
+    float psd[NSTR];
+    if constexpr (STAGE) {
+        const int e0 = blk * BLOCK + threadIdx.x;
+        if (s.eblk_model ? e0 < s.eblk_lim[blk] : e0 < s.E) {
+            pre[0] = s.pres[e0];
+            pre[1] = s.dpressure[e0];
+#pragma unroll
+            for (int k = 0; k < NSTR; ++k) psd[k] = s.sdev[k * (long)s.E + e0];
+        }
+    }
     const bool staged = STAGE && stage_block(s, blk, lds);
-    pass_e_mixed_body<ND>(s, fl, blk, staged, lds);
+    pass_e_mixed_body<ND>(s, fl, blk, staged, lds, STAGE ? pre : nullptr, STAGE ? psd : nullptr);
 }
 
 // node records from the SoA state, at the start of a fused run
