@@ -56,12 +56,12 @@ ELEM_PARAM_BYTES = {"fp64": 16, "mixed": 8}     # per-element viscosity and cohe
 # (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved): (2 FETCH + WRITE)
 # * 1024.  The commands and per-kernel tables are in profiles/.
 TRAFFIC = {
-    "3d-8m": {"k_pass_a": 818257671, "k_pass_b": 838479491, "k_pass_c": 2465313417, "k_pass_d": 372998176,
-              "k_pass_e": 2000112900, "k_pass_f": 1230323920},
-    "3d-fp64": {"k_pass_a": 92677773, "k_pass_b": 103475381, "k_pass_c": 294105556, "k_pass_d": 35105088,
-                "k_pass_e": 239736826, "k_pass_f": 152738806},
-    "3d-mixed": {"k_pass_a": 818264974, "k_pass_b": 838616656, "k_pass_c_mixed": 1681562193, "k_pass_d": 373508140,
-                 "k_pass_e_mixed": 1207732161, "k_pass_f32": 741462756},
+    "3d-8m": {"k_pass_a": 818267498, "k_pass_b": 827779418, "k_pass_c": 2374492480, "k_pass_d": 376022092,
+              "k_pass_e": 1844706954, "k_pass_f": 1232081672},
+    "3d-fp64": {"k_pass_a": 92687242, "k_pass_b": 102090951, "k_pass_c": 285688544, "k_pass_d": 35102912,
+                "k_pass_e": 221224904, "k_pass_f": 152412715},
+    "3d-mixed": {"k_pass_a": 818271327, "k_pass_b": 828551662, "k_pass_c_mixed": 1647330909, "k_pass_d": 375998541,
+                 "k_pass_e_mixed": 1125722361, "k_pass_f32": 739783821},
 }
 
 WORKLOADS = {
