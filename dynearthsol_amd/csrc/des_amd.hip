@@ -1596,21 +1596,29 @@ __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags 
     if (rot) jaumann<ND>(st, dtp, w);
     // stressyy exists only under plane strain (2D); nothing else reads or writes it
     double syy = (ND == 2 && s.P->is_plane_strain) ? s.stressyy[e] : 0.0, dp;
-    double pl = __builtin_nontemporal_load(s.plstrain + e), dpl = __builtin_nontemporal_load(s.delta_plstrain + e);
+    double pl = __builtin_nontemporal_load(s.plstrain + e), dpl = 0;
     if (fl.rec_plstrain) fl.rec_plstrain[e] = pl;
     if (fl.rec_volume_old) fl.rec_volume_old[e] = s.volume_old[e];
     bool ok = true, defer = false;
+    // which of plastic strain and its increment this pass changes (and stores)
+    bool store_pl = true, store_dpl = true;
     double trial[6], old_s = 0;
     const double Tmean = node_mean<ND>(Tn);
     if (DEFER && ND == 3 && (s.P->rheol_type == DES_RH_EP || s.P->rheol_type == DES_RH_EVP)) {
-        defer = plastic_deferred3(s, e, dt, Tmean, vol, s.volume_old[e], st, edot, pl, dpl, dp, trial, old_s);
+        // the plastic strain stays; its increment is zeroed or left as it is,
+        // so neither is read back to be stored unchanged
+        defer = plastic_deferred3(s, e, dt, Tmean, vol, s.volume_old[e], st, edot, pl, store_dpl, dp, trial,
+                                  old_s);
+        store_pl = false;
     } else if constexpr (!PLASTIC_ONLY) {
         // 3D plastic rheologies are deferred above: their inline return is not compiled in here
+        dpl = __builtin_nontemporal_load(s.delta_plstrain + e);
         ok = rheology_after_strain<ND, !(DEFER && ND == 3)>(s, e, dt, total_dv, Tmean, vol, s.volume_old[e], st,
                                                            syy, edot, pl, dpl, dp);
     } else {
         ok = false;        // launched for elasto-(visco-)plastic 3D models only
         dp = 0;
+        store_pl = store_dpl = false;
     }
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     if constexpr (DEFER) {
@@ -1634,8 +1642,8 @@ __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags 
     }
     store_tensor<ND>(s.stress, E, e, st);
     if (s.P->is_plane_strain) s.stressyy[e] = syy;
-    __builtin_nontemporal_store(pl, s.plstrain + e);
-    __builtin_nontemporal_store(dpl, s.delta_plstrain + e);
+    if (store_pl) __builtin_nontemporal_store(pl, s.plstrain + e);
+    if (store_dpl) __builtin_nontemporal_store(dpl, s.delta_plstrain + e);
     s.dpressure[e] = dp;
     s.escal[e] = dp * vol;          // NMD_stress's dp * volume (geometry.cxx:206)
 }
@@ -1962,14 +1970,18 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
 #pragma unroll
     for (int i = 0; i < NPE; ++i) edv += aux[i];
     edv = edv / NPE;
+    // in 3D (returns deferred) the update never changes the plastic strain
+    // and only zeroes its increment (reset_dpl): neither is read back to be
+    // stored unchanged
     double pl = __builtin_nontemporal_load(s.plstrain + e);
-    float dpl = __builtin_nontemporal_load(s.dpl32 + e);
+    float dpl = ND == 3 ? 0.f : __builtin_nontemporal_load(s.dpl32 + e);
+    bool reset_dpl;
     double dp;
     const float Tm = (float)node_mean<ND>(Tn);
     bool deferred;
     MixedRet rec;
     bool ok = stress_update_mixed<ND, ND == 3>(s, e, (float)s.sc->dt, (float)edv, Tm, vol, s.volume_old[e], p,
-                                               sd, es, wf, dtp, rot, edot, pl, dpl, dp, deferred, rec);
+                                               sd, es, wf, dtp, rot, edot, pl, dpl, reset_dpl, dp, deferred, rec);
     if (!ok) set_error(s.sc, DES_ERR_INPUT);
     if constexpr (ND == 3) {
         // append the deferred elements, one atomic per wavefront
@@ -1992,8 +2004,8 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
         s.pres[e] = p;
 #pragma unroll
         for (int k = 0; k < NSTR; ++k) s.sdev[k * E + e] = sd[k];
-        __builtin_nontemporal_store(pl, s.plstrain + e);
-        __builtin_nontemporal_store(dpl, s.dpl32 + e);
+        if (ND == 2) __builtin_nontemporal_store(pl, s.plstrain + e);
+        if (ND == 2 || reset_dpl) __builtin_nontemporal_store(dpl, s.dpl32 + e);
         s.dpressure[e] = dp;
         s.escal[e] = dp * vol;
     }

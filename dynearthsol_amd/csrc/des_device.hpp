@@ -1138,12 +1138,15 @@ template <int ND, bool PLASTIC = true> __device__ __forceinline__ bool rheology_
 // Returns true for such an element: s then holds the Maxwell stress (EVP) or
 // the trial stress (EP), trial and old_s what the return needs, and plstrain
 // and delta_plstrain are untouched.  Otherwise the element is complete with
-// the values rheology_after_strain gives it.
+// the values rheology_after_strain gives it: the plastic strain unchanged
+// (plstrain += 0) and, when the elastic trial is taken, delta_plstrain = 0
+// (reset_dpl; the Maxwell stress of an EVP element leaves it as it is).
 __device__ __forceinline__ bool plastic_deferred3(const DevState &st, int e, double dt, double Tmean,
                                                   double vol, double vol_old, double *s, const double *edot,
-                                                  double &plstrain, double &delta_plstrain,
+                                                  double plstrain, bool &reset_dpl,
                                                   double &dpressure, double *trial, double &old_s)
 {
+    reset_dpl = false;
     const des_params *P = st.P;
     old_s = trace<3>(s);
     double de[6];
@@ -1169,8 +1172,7 @@ __device__ __forceinline__ bool plastic_deferred3(const DevState &st, int e, dou
     if (!defer && !(evp && svII < second_invariant2<3>(trial))) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) s[i] = trial[i];
-        plstrain += 0.0;
-        delta_plstrain = 0;
+        reset_dpl = true;
     } else if (evp) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) s[i] = sv[i];
@@ -1382,16 +1384,19 @@ template <int ND> __device__ __forceinline__ void split_stress(const double *s, 
 // DEFER (3D): an elasto-(visco-)plastic element whose elastic trial is not
 // clearly elastic (clearly_elastic3) fills rec and sets deferred; p, sd, pl
 // and dpl are then untouched and the return kernel completes the element.
+// reset_dpl: dpl was zeroed (the elastic trial taken); in 3D nothing else
+// here changes pl or dpl.
 // Every other element gets exactly what the inline return gives it, since
 // the return leaves a clearly elastic trial unchanged.
 template <int ND, bool DEFER = false>
 __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, float dtf, float edvoldt,
                                                     float Tmean, double vol, double vol_old, double &p,
                                                     float *sd, float *es, const float *wf, float dtp, bool rot,
-                                                    float *edot, double &pl, float &dpl, double &dpressure,
-                                                    bool &deferred, MixedRet &rec)
+                                                    float *edot, double &pl, float &dpl, bool &reset_dpl,
+                                                    double &dpressure, bool &deferred, MixedRet &rec)
 {
     deferred = false;
+    reset_dpl = false;
     constexpr int NSTR = Dim<ND>::NSTR;
     const des_params *P = st.P;
     const double p_old = p;
@@ -1499,6 +1504,7 @@ __device__ __forceinline__ bool stress_update_mixed(const DevState &st, int e, f
 #pragma unroll
             for (int i = 0; i < NSTR; ++i) sd[i] += inc[i];
             dpl = 0;
+            reset_dpl = true;
         } else {
             split_stress<ND>(s64, p, sd);
             pl += depls;                 // accumulated in FP64: increments below an FP32 ulp count
