@@ -102,16 +102,17 @@ __device__ __forceinline__ bool stage_block(const DevState &s, int blk, double2 
     return true;
 }
 
-// the records of element e's nodes, from the staged block or from nrec
+// the records of element e's nodes, from the staged block (pli: the
+// element's stage_lidx entry when the caller loaded it) or from nrec
 template <int ND> __device__ __forceinline__ void load_elem_nodes(const DevState &s, int e, bool staged,
                                                                 const double2 *lds,
                                                                 double x[Dim<ND>::NPE][ND],
                                                                 double v[Dim<ND>::NPE][ND],
                                                                 double T[Dim<ND>::NPE],
-                                                                double aux[Dim<ND>::NPE])
+                                                                double aux[Dim<ND>::NPE], const unsigned *pli = nullptr)
 {
     if (staged) {
-        const unsigned li = s.stage_lidx[e];
+        const unsigned li = pli ? *pli : s.stage_lidx[e];
         const lds_ptr<double2> l = (lds_ptr<double2>)lds;
 #pragma unroll
         for (int i = 0; i < Dim<ND>::NPE; ++i)
@@ -498,18 +499,17 @@ template <int ND> __device__ __forceinline__ void force_contrib(const DevState &
 // The CSR positions (force slots) of element e's NPE entries: the start of
 // each node's list plus the entry's position in it, one byte each (srank),
 // with the list starts from the staged block's LDS (loff) or from csr_off;
-// the full 4-byte positions (slot) when some list is longer than 256, or
-// always with RANK false (the mixed schedule: 12-byte slots, measured faster
-// with the plain positions).
-template <int ND, bool RANK = true>
+// the full 4-byte positions (slot) when some list is longer than 256.  pli:
+// the element's stage_lidx and srank entries when the caller loaded them.
+template <int ND>
 __device__ __forceinline__ void force_slots(const DevState &s, int e, bool staged, const int *loff,
-                                            int pos[Dim<ND>::NPE])
+                                            int pos[Dim<ND>::NPE], const unsigned *pli = nullptr)
 {
     constexpr int NPE = Dim<ND>::NPE;
-    if (RANK && s.srank) {
-        const unsigned r = s.srank[e];
+    if (s.srank) {
+        const unsigned r = pli ? pli[1] : s.srank[e];
         if (staged) {
-            const unsigned li = s.stage_lidx[e];
+            const unsigned li = pli ? pli[0] : s.stage_lidx[e];
             const lds_ptr<int> l = (lds_ptr<int>)loff;
 #pragma unroll
             for (int i = 0; i < NPE; ++i) pos[i] = l[(li >> (8 * i)) & 255u] + (int)((r >> (8 * i)) & 255u);
@@ -1746,14 +1746,16 @@ template <int ND, int NB, int DEPTH = 4> __global__ __launch_bounds__(NB) void k
 // loff: with staged, the staged nodes' list starts (stage_block)
 template <int ND, bool EREC = false>
 __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds,
-                                            const int *loff, double2 *wbuf = nullptr, const double *pre = nullptr)
+                                            const int *loff, double2 *wbuf = nullptr, const double *pre = nullptr,
+                                            const unsigned *pli = nullptr)
 {
-    // pre: the element's stress and dpressure when the caller loaded them
+    // pre: the element's stress and dpressure, pli its stage_lidx and srank
+    // entries, when the caller loaded them
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
-    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn);
+    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn, staged ? pli : nullptr);
     double st[NSTR];
     if (pre) {
 #pragma unroll
@@ -1777,7 +1779,7 @@ __device__ __forceinline__ void pass_e_body(DevState s, const FusedFlags &fl, in
         store_force_rec<ND>(s, e, out, wbuf);
     } else {
         int pos[NPE];
-        force_slots<ND>(s, e, staged, loff, pos);
+        force_slots<ND>(s, e, staged, loff, pos, staged ? pli : nullptr);
         store_force_contrib<ND>(s, pos, out);
     }
 }
@@ -1787,21 +1789,24 @@ template <int ND, bool STAGE, int WPE = 1> __global__ __launch_bounds__(BLOCK, W
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     __shared__ int loff[STAGE ? STAGE_MAX : 1];
     const int blk = xcd_block(s.rev);
-    // the element's stress and pressure change do not depend on the staged
-    // records: requested before the block's staging barrier, so that their
-    // HBM latency overlaps the staging's (the compiler keeps loads written
-    // after a barrier after it)
+    // the element's stress and pressure change, its staged-node positions
+    // and its slot positions do not depend on the staged records: requested
+    // before the block's staging barrier, so that their HBM latency overlaps
+    // the staging's (the compiler keeps loads written after a barrier after it)
     constexpr int NSTR = Dim<ND>::NSTR;
     double pre[NSTR + 1];
+    unsigned pli[2] = {0u, 0u};
     if constexpr (STAGE) {
         const int e0 = blk * BLOCK + threadIdx.x;
         if (s.eblk_model ? e0 < s.eblk_lim[blk] : e0 < s.E) {
             load_tensor<ND>(s.stress, s.E, e0, pre);
             pre[NSTR] = s.dpressure[e0];
+            if (s.stage) pli[0] = s.stage_lidx[e0];
+            if (s.srank) pli[1] = s.srank[e0];
         }
     }
     const bool staged = STAGE && stage_block(s, blk, lds, loff);
-    pass_e_body<ND>(s, fl, blk, staged, lds, loff, nullptr, STAGE ? pre : nullptr);
+    pass_e_body<ND>(s, fl, blk, staged, lds, loff, nullptr, STAGE ? pre : nullptr, STAGE ? pli : nullptr);
 }
 
 // pass E writing element-major force records
@@ -2080,13 +2085,14 @@ __global__ __launch_bounds__(BLOCK) void k_pass_c_ret_mixed(DevState s)
 
 template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, const FusedFlags &fl, int blk,
                                                                   bool staged, const double2 *lds,
-                                                                  const double *pre = nullptr, const float *psd = nullptr)
+                                                                  const double *pre = nullptr, const float *psd = nullptr,
+                                                                  const unsigned *pli = nullptr, const int *loff = nullptr)
 {
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], v[NPE][ND], Tn[NPE], dpn[NPE];
-    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn);
+    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, dpn, staged ? pli : nullptr);
     const double dp_el = node_mean<ND>(dpn);
     double p = (pre ? pre[0] : s.pres[e]) + (-(pre ? pre[1] : s.dpressure[e]) + dp_el) / ND;
     s.pres[e] = p;
@@ -2100,19 +2106,21 @@ template <int ND> __device__ __forceinline__ void pass_e_mixed_body(DevState s, 
     shape_fn<ND>(x, vol, sh);
     force_contrib<ND>(s, e, vol, sh, st, node_mean<ND>(Tn), out);
     int pos[NPE];
-    force_slots<ND, false>(s, e, staged, nullptr, pos);
+    force_slots<ND>(s, e, staged, loff, pos, staged ? pli : nullptr);
     store_force_contrib_f32<ND>(s, pos, out);
 }
 
 template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_mixed(DevState s, FusedFlags fl)
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
+    __shared__ int loff[STAGE ? STAGE_MAX : 1];
     const int blk = xcd_block(s.rev);
-    // pressure, pressure change and deviator requested before the staging
-    // barrier (as in k_pass_e)
+    // pressure, pressure change, deviator, staged-node and slot positions
+    // requested before the staging barrier (as in k_pass_e)
     constexpr int NSTR = Dim<ND>::NSTR;
     double pre[2];
     float psd[NSTR];
+    unsigned pli[2] = {0u, 0u};
     if constexpr (STAGE) {
         const int e0 = blk * BLOCK + threadIdx.x;
         if (s.eblk_model ? e0 < s.eblk_lim[blk] : e0 < s.E) {
@@ -2120,10 +2128,13 @@ template <int ND, bool STAGE> __global__ __launch_bounds__(BLOCK) void k_pass_e_
             pre[1] = s.dpressure[e0];
 #pragma unroll
             for (int k = 0; k < NSTR; ++k) psd[k] = s.sdev[k * (long)s.E + e0];
+            if (s.stage) pli[0] = s.stage_lidx[e0];
+            if (s.srank) pli[1] = s.srank[e0];
         }
     }
-    const bool staged = STAGE && stage_block(s, blk, lds);
-    pass_e_mixed_body<ND>(s, fl, blk, staged, lds, STAGE ? pre : nullptr, STAGE ? psd : nullptr);
+    const bool staged = STAGE && stage_block(s, blk, lds, loff);
+    pass_e_mixed_body<ND>(s, fl, blk, staged, lds, STAGE ? pre : nullptr, STAGE ? psd : nullptr,
+                          STAGE ? pli : nullptr, loff);
 }
 
 // node records from the SoA state, at the start of a fused run

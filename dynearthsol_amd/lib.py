@@ -167,8 +167,8 @@ def set_force_layout(mode):
     """Force contributions of the fused FP64 schedule, process-wide: 0
     node-major CSR slots addressed by list start + one-byte position
     (default), 1 element-major records, 2 node-major slots addressed by their
-    full positions (as the mixed schedule's FP32 slots always are).  Results
-    do not depend on it."""
+    full positions (0 and 2 also for the mixed schedule's FP32 slots).
+    Results do not depend on it."""
     load().des_set_force_layout(int(mode))
 
 

@@ -176,8 +176,9 @@ void des_set_sweep_order(int mode);
    1 element-major records (an element's contributions to its nodes in one
    128-byte line in 3D, 64 bytes in 2D), written coalesced by pass E and
    gathered by pass F; 2 the node-major slots of 0 always found by their full
-   positions (as the mixed schedule's FP32 slots always are).  Pass F sums a
-   node's contributions in its CSR order every way: the same results. */
+   positions (this also for the mixed schedule's FP32 slots, which otherwise
+   are found as in 0).  Pass F sums a node's contributions in its CSR order
+   every way: the same results. */
 void des_set_force_layout(int mode);
 const char *des_last_error(const des_ctx *ctx);
 

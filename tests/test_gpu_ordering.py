@@ -196,6 +196,27 @@ def test_fused_run_is_independent_of_force_layout(name):
         assert out[0][1] == out[k][1]
 
 
+@pytest.mark.parametrize("name", ["evp3", "maxwell2", "mat3"])
+def test_mixed_run_is_independent_of_slot_addressing(name):
+    """The mixed schedule's FP32 slots, found from list start + one-byte
+    position (0) or by full positions (2)."""
+    from dynearthsol_amd.lib import DesModel, set_force_layout
+    m = SC.build(name)
+    print(name, m.nelem, "elements, mixed", flush=True)
+    out = []
+    for layout in (0, 2):
+        set_force_layout(layout)
+        try:
+            g = DesModel(m)
+            g.run(23, abi.DES_MIXED)
+            out.append((g.download(), g.scalars()))
+            g.close()
+        finally:
+            set_force_layout(0)
+    _same(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 def test_batch_is_independent_of_force_layout():
     from dynearthsol_amd.lib import DesBatch, set_force_layout
     models = [SC.build("evp3"), SC.build("bc2"), SC.build("maxwell3", nelem=1500)]
