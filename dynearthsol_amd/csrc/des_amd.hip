@@ -1538,8 +1538,12 @@ template <int ND, int DEPTH> __global__ __launch_bounds__(BLOCK) void k_pass_b(D
 
 template <int ND, bool DEFER, bool PLASTIC_ONLY = false>
 __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags &fl,
-                                                                          int e, bool staged, const double2 *lds)
+                                                                          int e, bool staged, const double2 *lds,
+                                            const unsigned *pli = nullptr, const double *pes = nullptr,
+                                            const double *pst = nullptr)
 {
+    // pli, pes, pst: the element's stage_lidx entry, strain and stress when
+    // the caller loaded them
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE], vol;
@@ -1548,7 +1552,7 @@ __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags 
         // (the same operations on the same values: the same bits) instead of
         // read back
         double x[NPE][ND];
-        load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, aux);
+        load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, aux, staged ? pli : nullptr);
         vol = elem_volume<ND>(x);
         shape_fn<ND>(x, vol, sh);
     }
@@ -1570,7 +1574,12 @@ __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags 
 #pragma unroll
         for (int i = 0; i < ND; ++i) edot[i] += (edv - div) / ND;
         double es[NSTR];
-        load_tensor_nt<ND>(s.strain, E, e, es);
+        if (pes) {
+#pragma unroll
+            for (int i = 0; i < NSTR; ++i) es[i] = pes[i];
+        } else {
+            load_tensor_nt<ND>(s.strain, E, e, es);
+        }
         {
             // rotated unconditionally and selected: a branch here keeps ~30
             // more registers live through the rheology below
@@ -1591,7 +1600,12 @@ __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags 
         s.edvoldt[e] = edv;
     }
     double st[NSTR];
-    load_tensor<ND>(s.stress, E, e, st);
+    if (pst) {
+#pragma unroll
+        for (int i = 0; i < NSTR; ++i) st[i] = pst[i];
+    } else {
+        load_tensor<ND>(s.stress, E, e, st);
+    }
     if (fl.rec_stress) store_tensor<ND>(fl.rec_stress, E, e, st);
     if (rot) jaumann<ND>(st, dtp, w);
     // stressyy exists only under plane strain (2D); nothing else reads or writes it
@@ -1649,10 +1663,12 @@ __device__ __forceinline__ void pass_c_body(const DevState &s, const FusedFlags 
 }
 
 template <int ND, bool PLASTIC_ONLY = false>
-__device__ __forceinline__ void pass_c_blk(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds)
+__device__ __forceinline__ void pass_c_blk(DevState s, const FusedFlags &fl, int blk, bool staged, const double2 *lds,
+                                           const unsigned *pli = nullptr, const double *pes = nullptr,
+                                           const double *pst = nullptr)
 {
     BIND_ELEM_AT(s, e, blk);
-    pass_c_body<ND, true, PLASTIC_ONLY>(s, fl, e, staged, lds);
+    pass_c_body<ND, true, PLASTIC_ONLY>(s, fl, e, staged, lds, pli, pes, pst);
 }
 
 // PLASTIC_ONLY: the instantiation for 3D contexts whose models are all
@@ -1664,8 +1680,22 @@ __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl
 {
     __shared__ double2 lds[STAGE ? STAGE_MAX * (NREC / 2) : 1];
     const int blk = xcd_block(s.rev);
+    // the staged variants request the element's staged-node positions,
+    // strain and stress before the staging barrier (as in k_pass_e)
+    constexpr int NSTR = Dim<ND>::NSTR;
+    unsigned pli = 0;
+    double pes[NSTR], pst[NSTR];
+    if constexpr (STAGE) {
+        const int e0 = blk * BLOCK + threadIdx.x;
+        if (s.eblk_model ? e0 < s.eblk_lim[blk] : e0 < s.E) {
+            if (s.stage) pli = s.stage_lidx[e0];
+            load_tensor_nt<ND>(s.strain, s.E, e0, pes);
+            load_tensor<ND>(s.stress, s.E, e0, pst);
+        }
+    }
     const bool staged = STAGE && stage_block(s, blk, lds);
-    pass_c_blk<ND, PLASTIC_ONLY>(s, fl, blk, staged, lds);
+    pass_c_blk<ND, PLASTIC_ONLY>(s, fl, blk, staged, lds, staged ? &pli : nullptr, STAGE ? pes : nullptr,
+                                 STAGE ? pst : nullptr);
 }
 
 // Pass C's Mohr-Coulomb returns (rheology.cxx:151-299, 594-635) of the 3D
