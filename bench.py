@@ -56,12 +56,12 @@ ELEM_PARAM_BYTES = {"fp64": 16, "mixed": 8}     # per-element viscosity and cohe
 # (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x the bytes moved): (2 FETCH + WRITE)
 # * 1024.  The commands and per-kernel tables are in profiles/.
 TRAFFIC = {
-    "3d-8m": {"k_pass_a": 818267498, "k_pass_b": 827779418, "k_pass_c": 2374492480, "k_pass_d": 376022092,
-              "k_pass_e": 1844706954, "k_pass_f": 1232081672},
-    "3d-fp64": {"k_pass_a": 92687242, "k_pass_b": 102090951, "k_pass_c": 285688544, "k_pass_d": 35102912,
-                "k_pass_e": 221224904, "k_pass_f": 152412715},
-    "3d-mixed": {"k_pass_a": 818271327, "k_pass_b": 828551662, "k_pass_c_mixed": 1647330909, "k_pass_d": 375998541,
-                 "k_pass_e_mixed": 1125722361, "k_pass_f32": 739783821},
+    "3d-8m": {"k_pass_a": 818262353, "k_pass_b": 828603477, "k_pass_c": 2127069184, "k_pass_d": 375919251,
+              "k_pass_e": 1821302892, "k_pass_f": 1230029760},
+    "3d-fp64": {"k_pass_a": 92678592, "k_pass_b": 102013877, "k_pass_c": 250262697, "k_pass_d": 35105536,
+                "k_pass_e": 216038982, "k_pass_f": 152488707},
+    "3d-mixed": {"k_pass_a": 818268498, "k_pass_b": 828633387, "k_pass_c_mixed": 1529733052, "k_pass_d": 375282400,
+                 "k_pass_e_mixed": 1044767783, "k_pass_f32": 737890883},
 }
 
 WORKLOADS = {
