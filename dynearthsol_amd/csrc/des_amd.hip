@@ -1967,13 +1967,20 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_from_mixed(DevState
     s.delta_plstrain[e] = (double)s.dpl32[e];
 }
 
-template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, const FusedFlags &fl, int blk)
+// staged, lds: the block's staged node records (k_pass_c_mixed's staged
+// variants); pli, psd, pes, pp: the element's stage_lidx entry, deviator,
+// strain and pressure when the caller loaded them
+template <int ND>
+__device__ __forceinline__ void pass_c_mixed_body(DevState s, const FusedFlags &fl, int blk, bool staged = false,
+                                                  const double2 *lds = nullptr, const unsigned *pli = nullptr,
+                                                  const float *psd = nullptr, const float *pes = nullptr,
+                                                  const double *pp = nullptr)
 {
     BIND_ELEM_AT(s, e, blk);
     constexpr int NPE = Dim<ND>::NPE, NSTR = Dim<ND>::NSTR;
     const long E = s.E;
     double x[NPE][ND], sh[ND][NPE], v[NPE][ND], Tn[NPE], aux[NPE];
-    load_elem_nodes<ND>(s, e, false, nullptr, x, v, Tn, aux);
+    load_elem_nodes<ND>(s, e, staged, lds, x, v, Tn, aux, staged ? pli : nullptr);
     const double vol = elem_volume<ND>(x);       // pass A's volume, recomputed (same bits)
     shape_fn<ND>(x, vol, sh);
     // velocities relative to node 0 (sum_i dN_i = 0): no cancellation of the
@@ -1982,12 +1989,12 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
     for (int i = NPE - 1; i >= 0; --i)
 #pragma unroll
         for (int d = 0; d < ND; ++d) v[i][d] -= v[0][d];
-    double p = s.pres[e];
+    double p = pp ? *pp : s.pres[e];
     float sd[NSTR], es[NSTR], edot[NSTR];
 #pragma unroll
     for (int k = 0; k < NSTR; ++k) {       // strain and plastic strain: non-temporal streams
-        sd[k] = s.sdev[k * E + e];
-        es[k] = __builtin_nontemporal_load(s.strain32 + k * E + e);
+        sd[k] = psd ? psd[k] : s.sdev[k * E + e];
+        es[k] = pes ? pes[k] : __builtin_nontemporal_load(s.strain32 + k * E + e);
     }
     // strain rate and spin from the same products, before the rotation, so
     // that the velocities and shape functions are no longer held across it
@@ -2051,9 +2058,33 @@ template <int ND> __device__ __forceinline__ void pass_c_mixed_body(DevState s, 
     }
 }
 
-template <int ND, int WPE> __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c_mixed(DevState s, FusedFlags fl)
+// STAGE: the block's node records staged in LDS, with the element's own data
+// requested before the staging barrier (as in k_pass_e)
+template <int ND, int WPE, bool STAGE = false>
+__global__ __launch_bounds__(BLOCK, WPE) void k_pass_c_mixed(DevState s, FusedFlags fl)
 {
-    pass_c_mixed_body<ND>(s, fl, xcd_block(s.rev));
+    if constexpr (STAGE) {
+        __shared__ double2 lds[STAGE_MAX * (NREC / 2)];
+        constexpr int NSTR = Dim<ND>::NSTR;
+        const int blk = xcd_block(s.rev);
+        unsigned pli = 0;
+        float psd[NSTR], pes[NSTR];
+        double pp = 0;
+        const int e0 = blk * BLOCK + threadIdx.x;
+        if (s.eblk_model ? e0 < s.eblk_lim[blk] : e0 < s.E) {
+            if (s.stage) pli = s.stage_lidx[e0];
+#pragma unroll
+            for (int k = 0; k < NSTR; ++k) {
+                psd[k] = s.sdev[k * (long)s.E + e0];
+                pes[k] = __builtin_nontemporal_load(s.strain32 + k * (long)s.E + e0);
+            }
+            pp = s.pres[e0];
+        }
+        const bool staged = stage_block(s, blk, lds);
+        pass_c_mixed_body<ND>(s, fl, blk, staged, lds, staged ? &pli : nullptr, psd, pes, &pp);
+    } else {
+        pass_c_mixed_body<ND>(s, fl, xcd_block(s.rev));
+    }
 }
 
 // The Mohr-Coulomb returns that the mixed pass C deferred (3D), completing
@@ -4000,7 +4031,7 @@ int ensure_mixed(des_ctx *ctx)
 // variant count at tuning step k) and keep the fastest mean.
 constexpr int TUNE_A = DES_TUNE_A, TUNE_C = DES_TUNE_C, TUNE_D = DES_TUNE_D, TUNE_E = DES_TUNE_E,
               TUNE_F = DES_TUNE_F, TUNE_B = DES_TUNE_B;
-constexpr int TUNE_NV[2][DES_NTUNE] = {{3, 6, 5, 3, 5, 3}, {3, 4, 5, 2, 5, 3}};
+constexpr int TUNE_NV[2][DES_NTUNE] = {{3, 6, 5, 3, 5, 3}, {3, 6, 5, 2, 5, 3}};
 constexpr int TUNE_DEFAULT[DES_NTUNE] = {0, 1, 3, 0, 3, 1};
 constexpr int TUNE_STEPS = 2 * DES_TUNE_MAXV - 2, TUNE_MIN_RUN = 16;
 constexpr int PASSC_VARIANTS = 6;
@@ -4284,14 +4315,23 @@ int fused_step(des_ctx *ctx, int k, int nsteps, bool mixed, PassTimer &tm)
     MARK(1);
     ARM(false);
     rc = tunable(TUNE_C, [&](int v) -> int {
-        if (mixed) {       // a minimum of 1..4 waves per SIMD (variants 0..3)
+        if (mixed) {       // a minimum of 1..4 waves per SIMD (variants 0..3), LDS-staged 1..2 (4, 5)
+            const dim3 g(nblocks(ctx->E)), b(BLOCK);
+#define PCM(W, ST)                                                                                  \
+    (ctx->nd == 3 ? hipExtLaunchKernelGGL((k_pass_c_mixed<3, W, ST>), g, b, 0, ctx->stream, ctx->ev_a,  \
+                                          ctx->ev_b, 0, ctx->s, fl)                                 \
+                  : hipExtLaunchKernelGGL((k_pass_c_mixed<2, W, ST>), g, b, 0, ctx->stream, ctx->ev_a,  \
+                                          ctx->ev_b, 0, ctx->s, fl))
             switch (v) {
-            case 1: LAUNCH_T2(k_pass_c_mixed, 2, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
-            case 2: LAUNCH_T2(k_pass_c_mixed, 3, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
-            case 3: LAUNCH_T2(k_pass_c_mixed, 4, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
-            default: LAUNCH_T2(k_pass_c_mixed, 1, BLOCK, nblocks(ctx->E), ctx->s, fl); break;
+            case 1: PCM(2, false); break;
+            case 2: PCM(3, false); break;
+            case 3: PCM(4, false); break;
+            case 4: PCM(1, true); break;
+            case 5: PCM(2, true); break;
+            default: PCM(1, false); break;
             }
-            return DES_OK;
+#undef PCM
+            return check_launch(ctx);
         }
         return launch_pass_c(ctx, fl, v);
     });

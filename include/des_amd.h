@@ -331,11 +331,10 @@ int des_sync(des_ctx *ctx);
                              1 gathered directly, 2 pass A direct with at
                              least 6 waves per SIMD, FP64 pass E staged with
                              at least 8 (the mixed pass E has 0 and 1);
-     DES_TUNE_C:             FP64 pass C with a minimum of 1..4 waves per SIMD
-                             (0..3), or of 1 or 2 with the element block's
-                             node records staged in LDS (4, 5); the mixed
-                             pass C with a minimum of 1..4 waves per SIMD
-                             (0..3);
+     DES_TUNE_C:             pass C (FP64 and mixed) with a minimum of 1..4
+                             waves per SIMD (0..3), or of 1 or 2 with the
+                             element block's node records staged in LDS
+                             (4, 5);
      DES_TUNE_D, DES_TUNE_F: node blocks of 256, 128 or 64 threads with 4
                              gathered entries in flight per node (0..2), or
                              256 threads with 8 or 16 in flight (3, 4);
