@@ -60,7 +60,7 @@ TRAFFIC = {
               "k_pass_e": 1821302892, "k_pass_f": 1230029760},
     "3d-fp64": {"k_pass_a": 92678592, "k_pass_b": 102013877, "k_pass_c": 250262697, "k_pass_d": 35105536,
                 "k_pass_e": 216038982, "k_pass_f": 152488707},
-    "3d-mixed": {"k_pass_a": 818268498, "k_pass_b": 828633387, "k_pass_c_mixed": 1529733052, "k_pass_d": 375282400,
+    "3d-mixed": {"k_pass_a": 818268498, "k_pass_b": 828633387, "k_pass_c_mixed": 1433182104, "k_pass_d": 375282400,
                  "k_pass_e_mixed": 1044767783, "k_pass_f32": 737890883},
 }
 
