@@ -1673,8 +1673,8 @@ __device__ __forceinline__ void pass_c_blk(DevState s, const FusedFlags &fl, int
 
 // PLASTIC_ONLY: the instantiation for 3D contexts whose models are all
 // elasto-(visco-)plastic, without the other rheologies' code (whose registers
-// would otherwise set the kernel's allocation: 119 instead of 131 VGPRs, four
-// waves per SIMD instead of three)
+// would otherwise set the kernel's allocation: 119 VGPRs direct and 107 staged
+// instead of 131 and 133, four waves per SIMD instead of three)
 template <int ND, int WPE, bool STAGE, bool PLASTIC_ONLY = false>
 __global__ __launch_bounds__(BLOCK, WPE) void k_pass_c(DevState s, FusedFlags fl)
 {
