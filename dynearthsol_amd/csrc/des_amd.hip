@@ -303,14 +303,19 @@ template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_elem(DevState 
 }
 
 // temperature update with its boundary conditions (fields.cxx:153-162)
-__device__ __forceinline__ double temperature_new(const DevState &s, int n, double T, double tdot,
-                                                  double dt, double tmass)
+__device__ __forceinline__ double temperature_new_f(const DevState &s, unsigned f, double T, double tdot,
+                                                    double dt, double tmass)
 {
     const des_params *P = s.P;
-    unsigned f = s.bcflag[n];
     if (f & BOUNDZ1) return P->surface_temperature;
     if ((f & BOUNDZ0) && !P->has_fixed_bottom_temperature) return P->mantle_temperature;
     return T - tdot * dt / tmass;
+}
+
+__device__ __forceinline__ double temperature_new(const DevState &s, int n, double T, double tdot,
+                                                  double dt, double tmass)
+{
+    return temperature_new_f(s, s.bcflag[n], T, tdot, dt, tmass);
 }
 
 template <int ND> __global__ __launch_bounds__(BLOCK) void k_temp_node(DevState s)
@@ -1445,6 +1450,12 @@ template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevStat
     }
     constexpr int NPE = Dim<ND>::NPE;
     const int thermal = s.P->has_thermal_diffusion;
+    double T_old = 0;
+    unsigned bcf = 0;
+    if (thermal && q == 0) {      // the node's own values, requested before the gather
+        T_old = __builtin_nontemporal_load(s.T + n);
+        bcf = s.bcflag[n];
+    }
     // Record quarters: (c0, c1), (c2, c3), (dj, m), (tm, V).  Lane 0 sums the
     // node's thermal term c_i (quarter i/2, .x or .y by i's parity) in accx;
     // lane 1 sums dj and m, lane 2 tm and V (accx, accy); lane 3 repeats
@@ -1504,7 +1515,7 @@ template <int ND, int DEPTH> __device__ __forceinline__ void pass_b_body(DevStat
     double *rn = s.nrec + (long)NREC * n;
     double Tnew = 0;
     if (thermal) {      // the SoA temperature: read only by this pass (the others read the records)
-        Tnew = temperature_new(s, n, __builtin_nontemporal_load(s.T + n), tdot, s.sc->dt, tm);
+        Tnew = temperature_new_f(s, bcf, T_old, tdot, s.sc->dt, tm);
         __builtin_nontemporal_store(Tnew, s.T + n);
     }
     // dvoldt goes to the records only: ntmp is left to pass D (NMD_stress's
@@ -1747,6 +1758,7 @@ template <int ND, int NB, int DEPTH> __device__ __forceinline__ void pass_d_body
 {
     BIND_NODE_NB_AT(s, n, NB, blk);
     constexpr int NPE = Dim<ND>::NPE;
+    const double vn = s.volume_n[n];       // requested before the gather
     double sum = 0;
     const int p1 = s.csr_off[n + 1];
     int p = s.csr_off[n];
@@ -1761,7 +1773,7 @@ template <int ND, int NB, int DEPTH> __device__ __forceinline__ void pass_d_body
         for (int j = 0; j < DEPTH; ++j) sum += v[j];
     }
     for (; p < p1; ++p) sum += s.escal[s.csr_ent[p] / NPE];
-    const double dpn = sum / s.volume_n[n];
+    const double dpn = sum / vn;
     __builtin_nontemporal_store(dpn, s.ntmp + n);          // read only at a sync
     s.nrec[(long)NREC * n + NrecSlot<ND>::AUX] = dpn;
     if (fl.rec_dpn) fl.rec_dpn[n] = dpn;
@@ -1860,9 +1872,14 @@ __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl, i
     if (fl.step_end && n % BLOCK == 0 &&
         (s.nblk_model ? (tb_ == 0 || s.nblk_model[tb_ - 1] != s.nblk_model[tb_]) : n == 0))
         step_end_scalars(s.sc, s.P->rheol_type);
-    double f[ND], v[ND];
+    double f[ND], v[ND], x0[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] = __builtin_nontemporal_load(s.vel + d * s.N + n);
+    // the node's own values, requested before the gather
+    const double m = s.mass[n];
+    const unsigned flag = s.bcflag[n];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) x0[d] = __builtin_nontemporal_load(s.coord + d * s.N + n);
     node_force<ND, F32, DEPTH, EREC>(s, n, f, v);
     if (fl.store_aux) {
 #pragma unroll
@@ -1872,17 +1889,15 @@ __device__ __forceinline__ void pass_f_body(DevState &s, const FusedFlags &fl, i
 #pragma unroll
         for (int d = 0; d < ND; ++d) fl.rec_force[d * s.N + n] = f[d];
     }
-    const double m = s.mass[n];
 #pragma unroll
     for (int d = 0; d < ND; ++d) v[d] += dt * f[d] / m;
-    unsigned flag = s.bcflag[n];
     if (flag & BOUND_ANY)
         if (!vbc_node<ND>(s, flag, v)) set_error(s.sc, DES_ERR_ASSERT);
     double xn[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         __builtin_nontemporal_store(v[d], s.vel + d * s.N + n);
-        xn[d] = __builtin_nontemporal_load(s.coord + d * s.N + n) + v[d] * dt;
+        xn[d] = x0[d] + v[d] * dt;
         __builtin_nontemporal_store(xn[d], s.coord + d * s.N + n);
     }
     // the record's x and v: its first ND 16-byte chunks (T and aux stay)
